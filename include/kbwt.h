@@ -1,0 +1,99 @@
+/* libkbwt_hip: kolmo's block-sorting path (forward suffix sort with BWT, primary index and sampled indexes, and the
+ * inverse BWT) on one MI355X (gfx950), behind kolmo's own C entry points.
+ *
+ * Drop-in boundary: the three kolmo_sais_* functions below have the signatures, argument meaning, validation and
+ * return codes of src/algo/sais/sais.h:13,17,20 (implemented in src/algo/sais/sais.c:12027-12082), so linking this
+ * library in place of sais.c leaves every caller (src/codecs/bwt_stream.c:52,73-74,233,294) unchanged and every
+ * codec-5 and codec-18 stream byte-identical. The output never depends on `threads`, as sais.h:5 requires.
+ *
+ * Return codes: 0 ok; -1 bad arguments or indexes out of range (sais.c:12029-12032, 12057-12066); -2 out of memory
+ * (host or device); -3 a HIP runtime error (no usable device, a failed launch). There is no CPU fallback inside this
+ * library: a device error is reported, on stderr and through kbwt_last_error(), never hidden.
+ *
+ * Every function is reentrant; calls from several host threads are serialised per device. */
+#ifndef KBWT_H
+#define KBWT_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define KBWT_API __attribute__((visibility("default")))
+#else
+#define KBWT_API
+#endif
+
+#define KBWT_OK 0
+#define KBWT_ERR_ARG (-1)
+#define KBWT_ERR_NOMEM (-2)
+#define KBWT_ERR_DEVICE (-3)
+
+/* ---- drop-in entry points (host pointers) ---- */
+
+/* Replaces kolmo_sais_bwt_aux (sais.h:13; sais.c:12027-12053). BWT of T[0..n) into U (T == U allowed) in libsais'
+   convention: U[0] = T[n-1], then T[SA[p]-1] for every suffix-array row p except the one holding suffix 0.
+   I[k] = 1 + rank of the suffix starting at k*r, k = 0..(n-1)/r; I[0] is the primary index. r is a power of two
+   >= 2. A (n int32 of caller scratch) is not used: its contents on return are unspecified, as in the reference. */
+KBWT_API int32_t kolmo_sais_bwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t r, int32_t *I, int32_t threads);
+
+/* Replaces kolmo_sais_unbwt_aux (sais.h:17; sais.c:12055-12077). Inverse of the above (T == U allowed). Every
+   I[t], t = 0..(n-1)/r, must lie in 1..n (else -1); r is n or a power of two >= 2. Only I[0] steers the decode: the
+   other samples are a decoding aid that a decoder may ignore (FORMAT.md 22.6), and this one does. A is unused. */
+KBWT_API int32_t kolmo_sais_unbwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t r, const int32_t *I,
+                             int32_t threads);
+
+/* Replaces kolmo_sais_unbwt (sais.h:20; sais.c:12079-12082): kolmo_sais_unbwt_aux with r = n and primary index i. */
+KBWT_API int32_t kolmo_sais_unbwt(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t i, int32_t threads);
+
+/* ---- device-resident forms (no counterpart in the reference: the inputs already sit in HBM) ----
+   dT, dU are device pointers of the current device; I is host memory; stream is a hipStream_t (NULL: the library's
+   own stream). They return when the result is complete. Same semantics and codes as the drop-in functions;
+   dT == dU is allowed. */
+KBWT_API int32_t kbwt_dev_bwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, int32_t *I, void *stream);
+KBWT_API int32_t kbwt_dev_unbwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, const int32_t *I, void *stream);
+
+/* ---- telemetry of the calling thread's last call ---- */
+#define KBWT_MAX_ROUNDS 40
+typedef struct {
+    int32_t n;                         /* block length */
+    int32_t sigma;                     /* distinct byte values */
+    int32_t bits_per_symbol;           /* w: bits of the dense symbol code */
+    int32_t h0;                        /* symbols in the first sort key */
+    int32_t rounds;                    /* doubling rounds after the first sort */
+    int32_t passes[KBWT_MAX_ROUNDS];   /* radix passes of round r (round 0 = the first sort) */
+    int64_t active[KBWT_MAX_ROUNDS];   /* m_r: suffixes sorted in round r (m_0 = n) */
+    int32_t splitters;                 /* inverse: sublists of the list ranking */
+    double ms_fwd, ms_inv;             /* device time of the last forward / inverse call (HIP events) */
+} kbwt_stats;
+KBWT_API int32_t kbwt_last_stats(kbwt_stats *out);
+
+/* ---- measurement (no counterpart in the reference) ----
+   kbwt_profile(1) clears and starts per-kernel timing: every launch is bracketed by HIP events on its own stream, and
+   kbwt_profile_read fills up to max records (name, summed device ms, launches, summed elements) and returns their
+   count. kbwt_profile(0) stops it. kbwt_dev_copy is a plain 16-byte-vector device copy (16-byte aligned pointers and
+   size), the reference workload for scaling hardware byte counters. */
+typedef struct {
+    char name[32];
+    double ms;
+    int64_t launches;
+    int64_t elems;
+} kbwt_kernel_time;
+KBWT_API void kbwt_profile(int32_t on);
+KBWT_API int32_t kbwt_profile_read(kbwt_kernel_time *out, int32_t max);
+KBWT_API int32_t kbwt_dev_copy(void *dst, const void *src, size_t bytes, void *stream);
+
+/* Message of the calling thread's last error ("" when none). */
+KBWT_API const char *kbwt_last_error(void);
+
+/* 1 when a gfx950 device is usable, else 0. */
+KBWT_API int32_t kbwt_device_ok(void);
+
+/* Releases the cached device workspace of the current device. */
+KBWT_API void kbwt_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,238 @@
+"""kolmo_amd: kolmo's block-sorting path (forward BWT with sampled indexes, inverse BWT) on MI355X.
+
+The product is the C-ABI library kolmo_amd/lib/libkbwt_hip.so (include/kbwt.h). This module is its host-side mirror
+for Python callers and tests. The raw entry points keep the names, argument meaning and return codes of
+src/algo/sais/sais.h:13,17,20 of kolmo 0.0.2 (kolmo_sais_bwt_aux, kolmo_sais_unbwt_aux, kolmo_sais_unbwt); the
+convenience wrappers bwt_aux / unbwt_aux raise KbwtError on a non-zero code.
+
+There is no CPU path here: loading fails loudly when the library is missing, and a call fails with code -3 when no
+usable device is present.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "lib")
+HIP_LIB = os.path.join(LIBDIR, "libkbwt_hip.so")
+GEN_LIB = os.path.join(LIBDIR, "libkbwt_gen.so")
+
+KBWT_OK, KBWT_ERR_ARG, KBWT_ERR_NOMEM, KBWT_ERR_DEVICE = 0, -1, -2, -3
+MAX_ROUNDS = 40
+
+
+class KbwtError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"kbwt error {code}: {msg}")
+        self.code = code
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32), ("sigma", ctypes.c_int32), ("bits_per_symbol", ctypes.c_int32),
+        ("h0", ctypes.c_int32), ("rounds", ctypes.c_int32), ("passes", ctypes.c_int32 * MAX_ROUNDS),
+        ("active", ctypes.c_int64 * MAX_ROUNDS), ("splitters", ctypes.c_int32),
+        ("ms_fwd", ctypes.c_double), ("ms_inv", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        r = self.rounds
+        return {"n": self.n, "sigma": self.sigma, "bits_per_symbol": self.bits_per_symbol, "h0": self.h0,
+                "rounds": r, "passes": list(self.passes[: r + 1]), "active": list(self.active[: r + 1]),
+                "splitters": self.splitters, "ms_fwd": self.ms_fwd, "ms_inv": self.ms_inv}
+
+
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 32), ("ms", ctypes.c_double), ("launches", ctypes.c_int64),
+                ("elems", ctypes.c_int64)]
+
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_lib = None
+_gen = None
+
+
+def lib():
+    """The loaded libkbwt_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(HIP_LIB):
+            raise ImportError(f"{HIP_LIB} is missing: build it with `python -m kolmo_amd.build`")
+        L = ctypes.CDLL(HIP_LIB)
+        for name in ("kolmo_sais_bwt_aux", "kolmo_sais_unbwt_aux"):
+            f = getattr(L, name)
+            f.restype = ctypes.c_int32
+        L.kolmo_sais_bwt_aux.argtypes = [_u8p, _u8p, _i32p, ctypes.c_int32, ctypes.c_int32, _i32p, ctypes.c_int32]
+        L.kolmo_sais_unbwt_aux.argtypes = [_u8p, _u8p, _i32p, ctypes.c_int32, ctypes.c_int32, _i32p, ctypes.c_int32]
+        L.kolmo_sais_unbwt.restype = ctypes.c_int32
+        L.kolmo_sais_unbwt.argtypes = [_u8p, _u8p, _i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
+        L.kbwt_dev_bwt_aux.restype = ctypes.c_int32
+        L.kbwt_dev_bwt_aux.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _i32p,
+                                       ctypes.c_void_p]
+        L.kbwt_dev_unbwt_aux.restype = ctypes.c_int32
+        L.kbwt_dev_unbwt_aux.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _i32p,
+                                         ctypes.c_void_p]
+        L.kbwt_last_stats.restype = ctypes.c_int32
+        L.kbwt_last_stats.argtypes = [ctypes.POINTER(Stats)]
+        L.kbwt_last_error.restype = ctypes.c_char_p
+        L.kbwt_device_ok.restype = ctypes.c_int32
+        L.kbwt_release.restype = None
+        L.kbwt_profile.restype = None
+        L.kbwt_profile.argtypes = [ctypes.c_int32]
+        L.kbwt_profile_read.restype = ctypes.c_int32
+        L.kbwt_profile_read.argtypes = [ctypes.POINTER(KernelTime), ctypes.c_int32]
+        L.kbwt_dev_copy.restype = ctypes.c_int32
+        L.kbwt_dev_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def gen_lib():
+    global _gen
+    if _gen is None:
+        if not os.path.exists(GEN_LIB):
+            raise ImportError(f"{GEN_LIB} is missing: build it with `python -m kolmo_amd.build`")
+        _gen = ctypes.CDLL(GEN_LIB)
+    return _gen
+
+
+def _ptr(a, t=_u8p):
+    return a.ctypes.data_as(t)
+
+
+def _check(rc):
+    if rc != 0:
+        raise KbwtError(rc, lib().kbwt_last_error().decode())
+
+
+def n_samples(n, r):
+    return (n - 1) // r + 1 if n > 0 else 1
+
+
+def bwt_aux(T, r):
+    """BWT of T (bytes or uint8 array) with sampling interval r: returns (U as uint8 array, I as int32 array)."""
+    T = np.ascontiguousarray(np.frombuffer(T, np.uint8) if isinstance(T, (bytes, bytearray)) else T, np.uint8)
+    n = T.size
+    U = np.empty(max(n, 1), np.uint8)
+    I = np.zeros(n_samples(n, r), np.int32)
+    A = np.empty(1, np.int32)   # caller scratch of the reference contract; unused by the device path
+    _check(lib().kolmo_sais_bwt_aux(_ptr(T), _ptr(U), _ptr(A, _i32p), n, r, _ptr(I, _i32p), 1))
+    return U[:n], I
+
+
+def unbwt_aux(U, r, I):
+    """Inverse BWT: returns the original bytes as a uint8 array."""
+    U = np.ascontiguousarray(np.frombuffer(U, np.uint8) if isinstance(U, (bytes, bytearray)) else U, np.uint8)
+    I = np.ascontiguousarray(I, np.int32)
+    n = U.size
+    T = np.empty(max(n, 1), np.uint8)
+    A = np.empty(1, np.int32)
+    _check(lib().kolmo_sais_unbwt_aux(_ptr(U), _ptr(T), _ptr(A, _i32p), n, r, _ptr(I, _i32p), 1))
+    return T[:n]
+
+
+def dev_bwt_aux(dT, r, dU=None, stream=None):
+    """Device-resident forward on a uint8 torch tensor of the current device: returns (dU, I)."""
+    import torch
+    n = dT.numel()
+    if dU is None:
+        dU = torch.empty_like(dT)
+    I = np.zeros(n_samples(n, r), np.int32)
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _check(lib().kbwt_dev_bwt_aux(dT.data_ptr(), dU.data_ptr(), n, r, _ptr(I, _i32p), s))
+    return dU, I
+
+
+def dev_unbwt_aux(dU, r, I, dT=None, stream=None):
+    import torch
+    n = dU.numel()
+    if dT is None:
+        dT = torch.empty_like(dU)
+    I = np.ascontiguousarray(I, np.int32)
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _check(lib().kbwt_dev_unbwt_aux(dU.data_ptr(), dT.data_ptr(), n, r, _ptr(I, _i32p), s))
+    return dT
+
+
+def last_stats():
+    st = Stats()
+    lib().kbwt_last_stats(ctypes.byref(st))
+    return st.as_dict()
+
+
+def device_ok():
+    return bool(lib().kbwt_device_ok())
+
+
+# ---- sampling-interval rules of the callers (src/codecs/bwt_stream.c:40-44 and :224-228) ----
+
+def bwt_mod(n):
+    """codec 5: the largest power of two <= n/8, minus one (0 when n < 16)."""
+    x = n // 8
+    if x <= 0:
+        return 0
+    return (1 << (x.bit_length() - 1)) - 1
+
+
+def codec18_rlog(n):
+    """codec 18 with index_log2 = 0: log2 of the sampling interval, at most 255 sampled indexes."""
+    r = bwt_mod(n) + 1
+    rlog = r.bit_length() - 1
+    rlog = max(rlog, 1)
+    while rlog < 30 and ((n - 1) >> rlog) > 255:
+        rlog += 1
+    return rlog
+
+
+# ---- seeded inputs ----
+
+GEN_KINDS = {"markov2": 0, "mixed": 1, "repeats": 2, "dna": 3, "random": 4, "csvcols": 5, "logs": 6, "bf16": 7,
+             "text": 8}
+
+
+def generate(kind, seed, n):
+    """Seeded synthetic block (see kolmo_amd/csrc/kbwt_gen.c for each generator)."""
+    g = gen_lib()
+    g.kbwt_gen_kind.restype = ctypes.c_int
+    g.kbwt_gen_kind.argtypes = [ctypes.c_int, ctypes.c_uint64, _u8p, ctypes.c_size_t]
+    out = np.empty(max(n, 1), np.uint8)
+    rc = g.kbwt_gen_kind(GEN_KINDS[kind] if isinstance(kind, str) else int(kind), seed, _ptr(out), n)
+    if rc != 0:
+        raise RuntimeError(f"generator {kind} failed ({rc})")
+    return out[:n]
+
+
+def generate_bsc_kind(kind, n):
+    """The input kinds of kolmo's tests/test_bwt.c:24-45."""
+    g = gen_lib()
+    g.kbwt_gen_bsc_kind.restype = ctypes.c_int
+    g.kbwt_gen_bsc_kind.argtypes = [ctypes.c_int, _u8p, ctypes.c_int]
+    out = np.empty(max(n, 1), np.uint8)
+    g.kbwt_gen_bsc_kind(kind, _ptr(out), n)
+    return out[:n]
+
+
+def batch_plan(seed, count, lo, hi, kinds):
+    g = gen_lib()
+    g.kbwt_gen_batch_plan.restype = ctypes.c_int
+    kinds = np.asarray([GEN_KINDS[k] if isinstance(k, str) else int(k) for k in kinds], np.int32)
+    sizes = np.empty(count, np.int64)
+    kout = np.empty(count, np.int32)
+    seeds = np.empty(count, np.uint64)
+    rc = g.kbwt_gen_batch_plan(ctypes.c_uint64(seed), ctypes.c_int(count), ctypes.c_int64(lo), ctypes.c_int64(hi),
+                               kinds.ctypes.data_as(ctypes.c_void_p), ctypes.c_int(len(kinds)),
+                               sizes.ctypes.data_as(ctypes.c_void_p), kout.ctypes.data_as(ctypes.c_void_p),
+                               seeds.ctypes.data_as(ctypes.c_void_p))
+    if rc != 0:
+        raise ValueError("bad batch plan")
+    return sizes, kout, seeds
+
+
+# The measured workloads: (generator, seed, n).
+WORKLOADS = {
+    "cfg1_markov2_1MiB": ("markov2", 1, 1 << 20),
+    "cfg2_mixed_32MiB": ("mixed", 2, 32 << 20),
+    "cfg3_repeats_128MiB": ("repeats", 3, 128 << 20),
+}

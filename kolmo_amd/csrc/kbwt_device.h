@@ -1,0 +1,133 @@
+// Device-side building blocks shared by the forward and inverse kernels (gfx950, wave64).
+//
+//  - decoupled-lookback status words: one 64-bit word per (tile, lane of interest) holding an epoch, a flag
+//    (aggregate or inclusive prefix) and a 32-bit value. The epoch makes a status array reusable by every launch
+//    without clearing it: a word written by an earlier launch carries an older epoch and reads as "not ready".
+//  - stable wave-level multisplit ranking of 8-bit digits with ballots (wave64 peers), the core of the radix passes
+//    and of the LF-map construction.
+//  - block scans for 256- or 512-thread workgroups.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kbwt {
+
+constexpr int WAVE = 64;
+constexpr uint32_t ST_AGG = 1u, ST_INC = 2u;
+constexpr uint32_t EPOCH_MASK = 0x3FFFFFFFu;
+
+__device__ __forceinline__ uint64_t st_pack(uint32_t epoch, uint32_t flag, uint32_t v) {
+    return ((uint64_t)(epoch & EPOCH_MASK) << 34) | ((uint64_t)flag << 32) | (uint64_t)v;
+}
+__device__ __forceinline__ void st_store(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t st_load(const uint64_t *p) {
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive prefix over tiles [0, tile) of the quantity whose status words sit at status[t * stride]. Sum or max.
+// Called by one thread per quantity. Every earlier tile has taken its tile id before this one (ids come from an
+// atomic counter), publishes its aggregate before it looks back itself, and so every wait here ends.
+template <bool MAX>
+__device__ __forceinline__ uint32_t lookback(const uint64_t *status, size_t stride, uint32_t tile, uint32_t epoch) {
+    uint32_t acc = 0;
+    int64_t t = (int64_t)tile - 1;
+    const uint32_t want = epoch & EPOCH_MASK;
+    while (t >= 0) {
+        uint64_t s = st_load(status + (size_t)t * stride);
+        uint32_t e = (uint32_t)(s >> 34), f = (uint32_t)(s >> 32) & 3u;
+        if (e != want || f == 0) { __builtin_amdgcn_s_sleep(1); continue; }
+        uint32_t v = (uint32_t)s;
+        acc = MAX ? (acc > v ? acc : v) : acc + v;
+        if (f == ST_INC) break;
+        --t;
+    }
+    return acc;
+}
+
+// Tile id from a per-launch counter; the workgroup that draws the last id resets it for the next launch (all
+// ntiles draws have happened by then).
+__device__ __forceinline__ uint32_t next_tile(uint32_t *counter, uint32_t ntiles) {
+    uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == ntiles - 1) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return t;
+}
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    uint32_t l = __lane_id();
+    return l ? (~0ull >> (64 - l)) : 0ull;
+}
+
+// Peers of this lane: the valid lanes whose 8-bit digit equals d. valid_mask = ballot(valid).
+__device__ __forceinline__ uint64_t match_digit8(uint32_t d, uint64_t valid_mask) {
+    uint64_t peers = valid_mask;
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        uint32_t bit = (d >> b) & 1u;
+        uint64_t bm = __ballot(bit);
+        peers &= bit ? bm : ~bm;
+    }
+    return peers;
+}
+
+// Wave inclusive scans with shuffles.
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t u = __shfl_up(v, o, WAVE);
+        if (lane >= (uint32_t)o) v += u;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        uint32_t u = __shfl_up(v, o, WAVE);
+        if (lane >= (uint32_t)o) v = v > u ? v : u;
+    }
+    return v;
+}
+
+// Block-wide exclusive sum and inclusive max of one value per thread, in thread order. scratch holds 2 * waves
+// words. Returns the block totals too. Contains barriers: every thread of the block must call it.
+template <int BLOCK>
+__device__ __forceinline__ void block_scan_sum_max(uint32_t vs, uint32_t vm, uint32_t &ex_sum, uint32_t &in_max,
+                                                   uint32_t &tot_sum, uint32_t &tot_max, uint32_t *scratch) {
+    constexpr int W = BLOCK / WAVE;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+    uint32_t is = wave_incl_sum(vs), im = wave_incl_max(vm);
+    if (lane == WAVE - 1) { scratch[wave] = is; scratch[W + wave] = im; }
+    __syncthreads();
+    uint32_t ps = 0, pm = 0, ts = 0, tm = 0;
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+        uint32_t s = scratch[w], m = scratch[W + w];
+        if (w < wave) { ps += s; pm = pm > m ? pm : m; }
+        ts += s; tm = tm > m ? tm : m;
+    }
+    __syncthreads();
+    ex_sum = ps + is - vs;
+    in_max = pm > im ? pm : im;
+    tot_sum = ts; tot_max = tm;
+}
+
+// Block-wide exclusive sum (one value per thread).
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t *scratch, uint32_t *total) {
+    constexpr int W = BLOCK / WAVE;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+    uint32_t is = wave_incl_sum(v);
+    if (lane == WAVE - 1) scratch[wave] = is;
+    __syncthreads();
+    uint32_t p = 0, t = 0;
+#pragma unroll
+    for (int w = 0; w < W; w++) { uint32_t s = scratch[w]; if (w < wave) p += s; t += s; }
+    __syncthreads();
+    if (total) *total = t;
+    return p + is - v;
+}
+
+}  // namespace kbwt
