@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="cfg2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="also run one plain device copy of this many bytes (the reference for byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): blocks are independent and a multi-GPU "
@@ -149,6 +151,15 @@ def main():
         if not parity:
             raise SystemExit("BWT differs from kolmo's reference output")
     st = K.last_stats()
+    if args.calib_copy:
+        src = torch.empty(args.calib_copy, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        src.fill_(7)
+        torch.cuda.synchronize()
+        rc = K.lib().kbwt_dev_copy(dst.data_ptr(), src.data_ptr(), args.calib_copy, stream.cuda_stream)
+        if rc != 0 or not torch.equal(src, dst):
+            raise SystemExit("calibration copy failed")
+        log(f"bench: calibration copy of {args.calib_copy} bytes done")
 
     log(f"bench: timing {args.steps} steps")
     L = K.lib()

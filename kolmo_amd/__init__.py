@@ -60,6 +60,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(HIP_LIB):
             raise ImportError(f"{HIP_LIB} is missing: build it with `python -m kolmo_amd.build`")
+        # One HIP runtime per process: torch wheels bundle their own libamdhip64 (same soname), and a second copy
+        # loaded next to ROCm's breaks device initialisation of whichever comes second. Loading torch first lets this
+        # library bind to the runtime torch already holds.
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
         L = ctypes.CDLL(HIP_LIB)
         for name in ("kolmo_sais_bwt_aux", "kolmo_sais_unbwt_aux"):
             f = getattr(L, name)

@@ -47,10 +47,11 @@ struct Workspace {
     hipStream_t stream = nullptr;
     std::mutex mu;
     uint32_t epoch = 1;
-    Buf T, U, K0, K1, V0, V1, SA, ISA, pos[2], grp[2], status, st_sum, st_max, partial, samples;
+    Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, st_max, samples, stage;
     // small fixed device buffers
     uint32_t *counters = nullptr;      // 16 tile counters (zeroed once)
-    uint32_t *counts = nullptr;        // 256
+    uint32_t *counts = nullptr;        // MAXP * 256 histogram accumulators, zero between uses
+    uint32_t *tot = nullptr;           // 256 byte counts
     uint32_t *dbase = nullptr;         // MAXP * 256
     uint32_t *cbase = nullptr;         // 256
     uint8_t *code = nullptr;           // 256
@@ -117,7 +118,10 @@ Workspace &workspace() {
         HIPCHK(hipMalloc(&w.counters, 16 * sizeof(uint32_t)));
         HIPCHK(hipMemset(w.counters, 0, 16 * sizeof(uint32_t)));
         HIPCHK(hipDeviceSynchronize());
-        HIPCHK(hipMalloc(&w.counts, 256 * sizeof(uint32_t)));
+        HIPCHK(hipMalloc(&w.counts, MAXP * 256 * sizeof(uint32_t)));
+        HIPCHK(hipMemset(w.counts, 0, MAXP * 256 * sizeof(uint32_t)));
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMalloc(&w.tot, 256 * sizeof(uint32_t)));
         HIPCHK(hipMalloc(&w.dbase, MAXP * 256 * sizeof(uint32_t)));
         HIPCHK(hipMalloc(&w.cbase, 256 * sizeof(uint32_t)));
         HIPCHK(hipMalloc(&w.code, 256));
@@ -181,9 +185,20 @@ inline uint32_t ceil_log2(uint32_t x) {
 }
 template <class T> T *P(Buf &b) { return reinterpret_cast<T *>(b.p); }
 
-uint32_t hist_grid(uint32_t n) { return std::max(1u, std::min<uint32_t>(HIST_GRID, (n + 4095) / 4096)); }
+uint32_t hist_grid(uint32_t n) { return std::max(1u, std::min<uint32_t>(HIST_GRID, (n + 8191) / 8192)); }
 
 #define TIMED(name, elems, call) timed(w, s, name, (uint64_t)(elems), [&] { call; })
+
+uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi) {
+    const char *v = getenv(name);
+    if (!v || !*v) return dflt;
+    long x = strtol(v, nullptr, 10);
+    return x < (long)lo ? lo : (x > (long)hi ? hi : (uint32_t)x);
+}
+
+// Clears the histogram accumulators: they are left at zero by every scan, and this makes a call that follows an
+// interrupted one start clean too.
+void reset_counts(Workspace &w, hipStream_t s) { launch_fill_u8(reinterpret_cast<uint8_t *>(w.counts), 0, MAXP * 256 * 4, s); }
 
 // ------------------------------------------------------------------ forward
 
@@ -198,28 +213,31 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t 
     grow(w.V1, N * 4);
     grow(w.SA, N * 4);
     grow(w.ISA, N * 4);
+    grow(w.L, N);
     for (int i = 0; i < 2; i++) { grow(w.pos[i], N * 4); grow(w.grp[i], N * 4); }
     const uint32_t ntiles_max = (uint32_t)((N + std::min(SORT_TILE, RG_TILE) - 1) / std::min(SORT_TILE, RG_TILE));
     grow(w.status, (size_t)ntiles_max * 256 * 8, true);
     grow(w.st_sum, (size_t)ntiles_max * 8, true);
     grow(w.st_max, (size_t)ntiles_max * 8, true);
-    grow(w.partial, (size_t)HIST_GRID * MAXP * 256 * 4);
     const uint32_t nsamp = (n - 1) / r + 1;
     grow(w.samples, (size_t)nsamp * 4);
 
     HIPCHK(hipEventRecord(w.ev0, s));
+    reset_counts(w, s);
     // 1. byte histogram -> dense symbol code
     const uint32_t g1 = hist_grid(n);
-    TIMED("byte_hist", n, launch_byte_hist(dT, n, P<uint32_t>(w.partial), g1, s));
-    TIMED("scan_digits", 256, launch_scan_digits(P<uint32_t>(w.partial), g1, 256, 0, 1, w.counts, nullptr, 0, s));
+    TIMED("byte_hist", n, launch_byte_hist(dT, n, w.counts, g1, s));
+    TIMED("scan_digits", 256, launch_scan_digits(w.counts, 1, w.tot, nullptr, 0, s));
     LAUNCHCHK();
-    HIPCHK(hipMemcpyAsync(w.h_counts, w.counts, 256 * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(w.h_counts, w.tot, 256 * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     uint32_t sigma = 0;
     for (int c = 0; c < 256; c++) { w.h_code[c] = (uint8_t)sigma; if (w.h_counts[c]) sigma++; }
     for (int c = 0; c < 256; c++) if (!w.h_counts[c]) w.h_code[c] = 0;
+    // 56 key bits above the BWT byte: 8 symbols when a symbol needs 5-7 bits, 7 bytes of binary data, and as many
+    // symbols as fit for small alphabets (14 of 4 bits, 18 of 3, 28 of 2, 56 of 1)
     const uint32_t wbits = sigma <= 2 ? 1 : ceil_log2(sigma);
-    const uint32_t h0 = wbits <= 4 ? 64 / wbits : 8;
+    const uint32_t h0 = wbits >= 5 ? std::min<uint32_t>(8, 56 / wbits) : 56 / wbits;
     const uint32_t P0 = (h0 * wbits + 7) / 8;
     st.sigma = (int32_t)sigma;
     st.bits_per_symbol = (int32_t)wbits;
@@ -227,15 +245,15 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t 
     HIPCHK(hipMemcpyAsync(w.code, w.h_code, 256, hipMemcpyHostToDevice, s));
 
     // 2. digit histograms of the first keys
-    TIMED("hist_T", n, launch_hist_T(dT, n, w.code, wbits, h0, P0, P<uint32_t>(w.partial), g1, s));
-    TIMED("scan_digits", 256 * P0,
-          launch_scan_digits(P<uint32_t>(w.partial), g1, MAXP * 256, 256, P0, nullptr, w.dbase, 0, s));
+    TIMED("hist_T", n, launch_hist_T(dT, n, w.code, wbits, h0, P0, w.counts, g1, s));
+    TIMED("scan_digits", 256 * P0, launch_scan_digits(w.counts, P0, nullptr, w.dbase, 0, s));
     LAUNCHCHK();
 
-    // 3. the first sort: P0 stable passes, the last one writing the suffix array
+    // 3. the first sort: P0 stable passes over the digits above the BWT byte, the last one writing the suffix array
     uint64_t *K[2] = {P<uint64_t>(w.K0), P<uint64_t>(w.K1)};
     uint32_t *V[2] = {P<uint32_t>(w.V0), P<uint32_t>(w.V1)};
     uint32_t *SA = P<uint32_t>(w.SA), *ISA = P<uint32_t>(w.ISA);
+    uint8_t *L = P<uint8_t>(w.L);
     for (uint32_t p = 0; p < P0; p++) {
         SortArgs a{};
         a.from_t = p == 0;
@@ -244,7 +262,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t 
         a.kout = K[p & 1];
         a.vout = p == P0 - 1 ? SA : V[p & 1];
         a.m = n;
-        a.shift = 8 * p;
+        a.shift = 8 * p + 8;
         a.dbase = w.dbase + p * 256;
         a.status = P<uint64_t>(w.status);
         a.counter = w.counters + 0;
@@ -259,10 +277,16 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t 
     st.passes[0] = (int32_t)P0;
     st.active[0] = n;
 
-    // 4. groups after the first sort
-    TIMED("regroup", n,
-          launch_regroup(K[(P0 - 1) & 1], SA, nullptr, n, nullptr, ISA, P<uint32_t>(w.pos[0]), P<uint32_t>(w.grp[0]),
-                         w.m_out, P<uint64_t>(w.st_sum), P<uint64_t>(w.st_max), w.counters + 1, w.epoch++, s));
+    // 4. groups after the first sort; the BWT byte of every row is its key's low byte
+    {
+        RegroupArgs g{};
+        g.K = K[(P0 - 1) & 1]; g.V = SA; g.pos = nullptr; g.m = n; g.kshift = 8;
+        g.SA = nullptr; g.ISA = ISA; g.L = L; g.T = nullptr;
+        g.pos_out = P<uint32_t>(w.pos[0]); g.grp_out = P<uint32_t>(w.grp[0]); g.m_out = w.m_out;
+        g.st_sum = P<uint64_t>(w.st_sum); g.st_max = P<uint64_t>(w.st_max); g.counter = w.counters + 1;
+        g.epoch = w.epoch++;
+        TIMED("regroup", n, launch_regroup(g, s));
+    }
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -289,9 +313,8 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t 
         int src = 0;
         if (sh.P) {
             const uint32_t g2 = hist_grid(m);
-            TIMED("hist_keys", m, launch_hist_keys(K[0], m, sh, P<uint32_t>(w.partial), g2, s));
-            TIMED("scan_digits", 256 * sh.P,
-                  launch_scan_digits(P<uint32_t>(w.partial), g2, MAXP * 256, 256, sh.P, nullptr, w.dbase, 0, s));
+            TIMED("hist_keys", m, launch_hist_keys(K[0], m, sh, w.counts, g2, s));
+            TIMED("scan_digits", 256 * sh.P, launch_scan_digits(w.counts, sh.P, nullptr, w.dbase, 0, s));
             LAUNCHCHK();
             for (uint32_t p = 0; p < sh.P; p++) {
                 SortArgs a{};
@@ -311,10 +334,15 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t 
                 src ^= 1;
             }
         }
-        TIMED("regroup", m,
-              launch_regroup(K[src], V[src], P<uint32_t>(w.pos[cur]), m, SA, ISA, P<uint32_t>(w.pos[cur ^ 1]),
-                             P<uint32_t>(w.grp[cur ^ 1]), w.m_out, P<uint64_t>(w.st_sum), P<uint64_t>(w.st_max),
-                             w.counters + 1, w.epoch++, s));
+        {
+            RegroupArgs g{};
+            g.K = K[src]; g.V = V[src]; g.pos = P<uint32_t>(w.pos[cur]); g.m = m; g.kshift = 0;
+            g.SA = SA; g.ISA = ISA; g.L = L; g.T = dT;
+            g.pos_out = P<uint32_t>(w.pos[cur ^ 1]); g.grp_out = P<uint32_t>(w.grp[cur ^ 1]); g.m_out = w.m_out;
+            g.st_sum = P<uint64_t>(w.st_sum); g.st_max = P<uint64_t>(w.st_max); g.counter = w.counters + 1;
+            g.epoch = w.epoch++;
+            TIMED("regroup", m, launch_regroup(g, s));
+        }
         LAUNCHCHK();
         HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
@@ -326,8 +354,8 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t 
     }
     st.rounds = round;
 
-    // 6. BWT bytes and sampled indexes
-    TIMED("bwt", n, launch_bwt(dT, SA, ISA, dU, n, s));
+    // 6. BWT bytes (the row bytes shifted around the primary row) and sampled indexes
+    TIMED("bwt", n, launch_bwt(dT, L, ISA, dU, n, s));
     TIMED("samples", nsamp, launch_samples(ISA, r, nsamp, P<int32_t>(w.samples), s));
     LAUNCHCHK();
     HIPCHK(hipEventRecord(w.ev1, s));
@@ -341,34 +369,38 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t 
 
 // ------------------------------------------------------------------ inverse
 
-constexpr uint32_t INV_DSHIFT = 6;   // one splitter row in 64
-
 void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, uint32_t n, uint32_t Pidx, hipStream_t s) {
     kbwt_stats &st = g_stats;
+    // one splitter row in D = 2^dshift; the first cap bytes of each sublist are staged during the counting walk
+    const uint32_t dshift = env_u32("KBWT_INV_DSHIFT", 6, 2, 12);
+    const uint32_t cap = env_u32("KBWT_INV_CAP", 4u << dshift, 1, 1u << 16);
     const size_t N = std::max<size_t>(n, 64);
+    const uint32_t dmask = (1u << dshift) - 1u;
+    const uint32_t nreg = (n >> dshift) + 1;
+    const uint32_t S = nreg + ((Pidx & dmask) != 0 ? 1 : 0);
     grow(w.V0, N * 4);                                 // lf
-    grow(w.pos[0], N * 4);                             // sublist lengths
-    grow(w.K0, N * 8);                                 // links (ping)
-    grow(w.K1, N * 8);                                 // links (pong)
+    grow(w.pos[0], ((size_t)S + 1) * 4);               // sublist lengths
+    grow(w.pos[1], ((size_t)S + 1) * 4);               // continuation rows
+    grow(w.K0, ((size_t)S + 1) * 8);                   // links (ping)
+    grow(w.K1, ((size_t)S + 1) * 8);                   // links (pong)
+    grow(w.stage, (size_t)S * cap);
     const uint32_t ntiles = (uint32_t)((N + LF_TILE - 1) / LF_TILE);
     grow(w.status, (size_t)ntiles * 256 * 8, true);
-    grow(w.partial, (size_t)HIST_GRID * MAXP * 256 * 4);
 
     HIPCHK(hipEventRecord(w.ev0, s));
+    reset_counts(w, s);
     const uint32_t g1 = hist_grid(n);
-    TIMED("byte_hist", n, launch_byte_hist(dL, n, P<uint32_t>(w.partial), g1, s));
-    TIMED("scan_digits", 256, launch_scan_digits(P<uint32_t>(w.partial), g1, 256, 0, 1, nullptr, w.cbase, 1, s));
+    TIMED("byte_hist", n, launch_byte_hist(dL, n, w.counts, g1, s));
+    TIMED("scan_digits", 256, launch_scan_digits(w.counts, 1, nullptr, w.cbase, 1, s));
     uint32_t *lf = P<uint32_t>(w.V0);
     TIMED("lf_pass", n, launch_lf_pass(dL, n, w.cbase, lf, P<uint64_t>(w.status), w.counters + 2, w.epoch++, s));
     LAUNCHCHK();
 
-    const uint32_t dmask = (1u << INV_DSHIFT) - 1u;
-    const uint32_t nreg = (n >> INV_DSHIFT) + 1;
-    const uint32_t S = nreg + ((Pidx & dmask) != 0 ? 1 : 0);
     st.splitters = (int32_t)S;
     uint64_t *link[2] = {P<uint64_t>(w.K0), P<uint64_t>(w.K1)};
-    uint32_t *len = P<uint32_t>(w.pos[0]);
-    TIMED("walk_count", n, launch_walk_count(lf, n, Pidx, INV_DSHIFT, nreg, S, link[0], len, s));
+    uint32_t *len = P<uint32_t>(w.pos[0]), *cont = P<uint32_t>(w.pos[1]);
+    uint8_t *stage = P<uint8_t>(w.stage);
+    TIMED("walk_count", n, launch_walk_count(lf, n, Pidx, dshift, nreg, S, w.cbase, link[0], len, stage, cap, cont, s));
     LAUNCHCHK();
     const uint32_t iters = ceil_log2(S + 1) + 1;
     int cur = 0;
@@ -381,7 +413,8 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, uint32_t n, uint32_
     HIPCHK(hipStreamSynchronize(s));
     const uint64_t l0 = *w.h_link;
     const bool valid = (uint32_t)l0 == S && (uint32_t)(l0 >> 32) == n;
-    if (valid) TIMED("walk_emit", n, launch_walk_emit(lf, n, Pidx, INV_DSHIFT, nreg, S, link[cur], len, w.cbase, dOut, s));
+    if (valid)
+        TIMED("walk_emit", n, launch_walk_emit(lf, n, Pidx, S, link[cur], len, w.cbase, stage, cap, cont, dOut, s));
     else launch_fill_u8(dOut, 0, n, s);   // not a BWT: deterministic bytes that fail the caller's checksum
     LAUNCHCHK();
     HIPCHK(hipEventRecord(w.ev1, s));
@@ -473,7 +506,7 @@ int32_t kbwt_dev_bwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, i
             return;
         }
         const uint8_t *src = dT;
-        if (dT == dU) {   // the BWT reads T at random while writing U: work from a copy
+        if (dT == dU || ((uintptr_t)dT & 15)) {   // in place, or not 16-byte aligned: work from a copy
             grow(w.T, (size_t)n + 64);
             HIPCHK(hipMemcpyAsync(w.T.p, dT, (size_t)n, hipMemcpyDeviceToDevice, s));
             src = P<uint8_t>(w.T);
@@ -497,7 +530,13 @@ int32_t kbwt_dev_unbwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r,
             HIPCHK(hipStreamSynchronize(s));
             return;
         }
-        inverse(w, dT, dU, (uint32_t)n, (uint32_t)I[0], s);
+        const uint8_t *src = dT;
+        if ((uintptr_t)dT & 15) {   // the byte histogram reads whole words: work from an aligned copy
+            grow(w.T, (size_t)n + 64);
+            HIPCHK(hipMemcpyAsync(w.T.p, dT, (size_t)n, hipMemcpyDeviceToDevice, s));
+            src = P<uint8_t>(w.T);
+        }
+        inverse(w, src, dU, (uint32_t)n, (uint32_t)I[0], s);
     });
 }
 
@@ -556,8 +595,8 @@ void kbwt_release(void) {
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
     Workspace &w = g_ws[dev];
     std::lock_guard<std::mutex> lk(w.mu);
-    Buf *bufs[] = {&w.T, &w.U, &w.K0, &w.K1, &w.V0, &w.V1, &w.SA, &w.ISA, &w.pos[0], &w.pos[1],
-                   &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.st_max, &w.partial, &w.samples};
+    Buf *bufs[] = {&w.T, &w.U, &w.K0, &w.K1, &w.V0, &w.V1, &w.SA, &w.ISA, &w.L, &w.pos[0], &w.pos[1],
+                   &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.st_max, &w.samples, &w.stage};
     for (Buf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;
