@@ -1,19 +1,24 @@
 #!/usr/bin/env python3
 """Forward + inverse BWT throughput of kolmo's block-sorting path on one MI355X.
 
-A step is one forward BWT (suffix sort, BWT bytes, primary and sampled indexes) plus one inverse BWT of one block
-already resident in HBM, through the library's device-resident entry points (kbwt_dev_bwt_aux / kbwt_dev_unbwt_aux).
-Default workload: one 32 MiB block (seed 2) of 70% order-3 Markov text over 96 symbols (Dirichlet(0.2) rows) and 30%
-CSV lines of 8 int32 random-walk columns, sampling interval r = 2^22 (codec 18's rule). Output: one JSON line.
+A step is one forward BWT (suffix sort, BWT bytes, primary and sampled indexes) plus one inverse BWT of the workload,
+already resident in HBM, through the library's device-resident entry points. Output: one JSON line.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload NAME] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload cfg2|cfg1|cfg3|cfg4] [--no-cpu]
 
-Besides the throughput it reports, for the dominant kernel, the algorithmic bytes per launch over its average
-duration measured with HIP events on its own stream inside the timed region (roofline), and kolmo's own CPU path
-(oracle/_ref, compiled from kolmo's sources) timed on this host's cores on the same block (cpu_baseline).
+Workloads (sampling interval r by codec 18's rule for every block):
+  cfg2 (default)  one 32 MiB block (seed 2): 70% order-3 Markov text over 96 symbols (Dirichlet(0.2) rows) and 30%
+                  CSV lines of 8 int32 random-walk columns
+  cfg1            one 1 MiB block (seed 1): order-2 Markov over 64 symbols (Dirichlet(0.3) rows)
+  cfg3            one 128 MiB deep-repeat block (seed 3)
+  cfg4            512 independent blocks (seed 4), sizes log-uniform 4 KiB-16 MiB, each from the cfg1 or cfg2
+                  generators, uniform bytes or DNA; batched calls, timed against one call per block
+
+Besides throughput it reports, for the dominant kernel, the algorithmic bytes per launch over its average duration
+measured with HIP events on its own stream inside the timed region (roofline), and kolmo's own CPU path
+(oracle/_ref, compiled from kolmo's sources) timed on this host's cores on a bounded sample of the same workload.
 """
 import argparse
-import ctypes
 import json
 import os
 import re
@@ -32,29 +37,30 @@ KERNEL_BYTES = {
     "hist_T": 1.0,         # read T
     "byte_hist": 1.0,
     "hist_keys": 8.0,      # read key
-    "regroup": 16.0,       # read key 8 + value 4, write ISA 4 (compaction writes of the few active rows not counted)
+    "regroup": 18.0,       # read key 8 + value 4, write ISA 4, SA row byte 1 (+ compaction of the active rows)
     "round_keys": 32.0,    # read pos, grp, SA, ISA[i+h] (4 each), write key 8 + value 4, plus the active row's SA read
-    "bwt": 6.0,            # read SA 4 + T 1, write U 1
+    "bwt": 2.0,            # read the row byte, write the BWT byte
     "lf_pass": 5.0,        # read L 1, write LF 4
-    "walk_count": 4.0,     # one LF read per step
-    "walk_emit": 5.0,      # one LF read and one byte write per step
+    "walk_count": 5.0,     # one LF read and one staged byte per step
+    "walk_emit": 2.0,      # one staged byte read and written per step
     "rank_jump": 24.0,     # read two links, write one
+    "fill_blk": 2.0,
 }
+
+WORKLOADS = {
+    "cfg1": ("markov2", 1, 1 << 20, "one 1 MiB block (seed 1): order-2 Markov over 64 symbols, Dirichlet(0.3)"),
+    "cfg2": ("mixed", 2, 32 << 20, "one 32 MiB block (seed 2): 70% order-3 Markov text (96 symbols, Dirichlet(0.2)) "
+             "interleaved with 30% CSV lines of 8 int32 random-walk columns (N(0,50) steps)"),
+    "cfg3": ("repeats", 3, 128 << 20, "one 128 MiB block (seed 3): 1 MiB random, then copies of earlier substrings "
+             "(64 B-4 MiB, 0.01% substitutions) and one 2 MiB single-byte run"),
+}
+BATCH = {"cfg4": (4, 512, 4096, 16 << 20, ["markov2", "mixed", "random", "dna"],
+                  "512 blocks (seed 4), sizes log-uniform 4 KiB-16 MiB, from the order-2 Markov and text+CSV "
+                  "generators, uniform bytes and DNA with N and tandem repeats")}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def workloads():
-    import kolmo_amd as K
-    return {
-        "cfg2": ("mixed", 2, 32 << 20, "one 32 MiB block (seed 2): 70% order-3 Markov text (96 symbols, Dirichlet(0.2)) "
-                 "interleaved with 30% CSV lines of 8 int32 random-walk columns (N(0,50) steps)"),
-        "cfg1": ("markov2", 1, 1 << 20, "one 1 MiB block (seed 1): order-2 Markov over 64 symbols, Dirichlet(0.3)"),
-        "cfg3": ("repeats", 3, 128 << 20, "one 128 MiB block (seed 3): 1 MiB random, then copies of earlier substrings "
-                 "(64 B-4 MiB, 0.01% substitutions) and one 2 MiB single-byte run"),
-    }, K
 
 
 def traffic_from_profiles(kernel):
@@ -76,8 +82,9 @@ def traffic_from_profiles(kernel):
     return None
 
 
-def cpu_baseline(T, r):
-    """kolmo's own kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux (oracle/_ref, built from kolmo's C), on this host."""
+def cpu_baseline(blocks, rs, budget_s=20.0, reps=2):
+    """kolmo's own kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux (oracle/_ref, built from kolmo's C) on this host's cores,
+    over the blocks in order until about budget_s seconds of work (each block best of `reps`)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import refs
@@ -86,44 +93,85 @@ def cpu_baseline(T, r):
     R = refs.ref()
     threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     threads = max(1, min(threads, 64))
-    n = T.size
-    best = None
-    for rep in range(2):
-        t0 = time.perf_counter()
-        rc, U, I = refs.sais_bwt(R, T, r, threads=threads)
-        t1 = time.perf_counter()
-        rc2, back = refs.sais_unbwt(R, U, r, I, threads=threads)
-        t2 = time.perf_counter()
-        assert rc == 0 and rc2 == 0 and np.array_equal(back, T)
-        if best is None or t2 - t0 < best[0]:
-            best = (t2 - t0, t1 - t0, t2 - t1)
-    return {"value": round(n / best[0] / 2**20, 2), "unit": "MiB/s", "cores": threads, "kind": "reference",
-            "sample": f"the full {n >> 20} MiB block, kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux (kolmo 0.0.2 sais.c, "
-                      f"gcc -O3 -march=x86-64-v3) at threads={threads}, best of 2; fwd {best[1]*1e3:.0f} ms, "
-                      f"inv {best[2]*1e3:.0f} ms"}
+    done_bytes, done_s, fwd_s, inv_s, used = 0, 0.0, 0.0, 0.0, 0
+    for T, r in zip(blocks, rs):
+        best = None
+        for rep in range(reps):
+            t0 = time.perf_counter()
+            rc, U, I = refs.sais_bwt(R, T, r, threads=threads)
+            t1 = time.perf_counter()
+            rc2, back = refs.sais_unbwt(R, U, r, I, threads=threads)
+            t2 = time.perf_counter()
+            assert rc == 0 and rc2 == 0 and np.array_equal(back, T)
+            if best is None or t2 - t0 < best[0]:
+                best = (t2 - t0, t1 - t0, t2 - t1)
+        done_bytes += T.size
+        done_s += best[0]
+        fwd_s += best[1]
+        inv_s += best[2]
+        used += 1
+        if done_s * reps >= budget_s:
+            break
+    what = (f"the full {blocks[0].size >> 20} MiB block" if len(blocks) == 1 else
+            f"the first {used} of {len(blocks)} blocks ({done_bytes / 2**20:.1f} MiB)")
+    return {"value": round(done_bytes / done_s / 2**20, 2), "unit": "MiB/s", "cores": threads, "kind": "reference",
+            "sample": f"{what}, kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux (kolmo 0.0.2 sais.c, gcc -O3 "
+                      f"-march=x86-64-v3) at threads={threads}, best of {reps}; fwd {fwd_s * 1e3:.0f} ms, "
+                      f"inv {inv_s * 1e3:.0f} ms"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cfg2")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--calib-copy", type=int, default=0,
-                    help="also run one plain device copy of this many bytes (the reference for byte counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): blocks are independent and a multi-GPU "
-              "run is not part of this build", file=sys.stderr)
-        sys.exit(2)
+def roofline_from(kernels, steps, ms_step):
+    dom = max(kernels, key=lambda k: kernels[k]["ms"])
+    d = kernels[dom]
+    avg_ms = d["ms"] / d["launches"]
+    bytes_per_launch = KERNEL_BYTES.get(dom, 0.0) * d["elems"] / d["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom), "kernel": dom,
+                "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
+    path_bytes = sum(KERNEL_BYTES.get(k, 0.0) * v["elems"] for k, v in kernels.items()) / steps
+    path = {"bytes_per_step": int(path_bytes), "achieved_GBs": round(path_bytes / (ms_step * 1e-3) / 1e9, 1),
+            "frac": round(path_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "kernel_ms_per_step": {k: round(v["ms"] / steps, 4) for k, v in sorted(kernels.items(),
+                                   key=lambda kv: -kv[1]["ms"])}}
+    return roofline, path, dom, achieved
 
-    import numpy as np
+
+def timed_steps(K, steps, step):
+    """Run `steps` steps with per-kernel profiling on; returns (seconds, {kernel: {ms, launches, elems}})."""
     import torch
-    W, K = workloads()
-    gen, seed, n, desc = W[args.workload]
-    if not K.device_ok():
-        raise SystemExit("no usable gfx950 device (or libkbwt_hip.so missing)")
+    L = K.lib()
+    L.kbwt_profile(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    recs = (K.KernelTime * 64)()
+    cnt = L.kbwt_profile_read(recs, 64)
+    L.kbwt_profile(0)
+    kernels = {recs[i].name.decode(): {"ms": recs[i].ms, "launches": recs[i].launches, "elems": recs[i].elems}
+               for i in range(cnt)}
+    return t1 - t0, kernels
+
+
+def calib_copy(K, nbytes, dev, stream):
+    import torch
+    src = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    src.fill_(7)
+    torch.cuda.synchronize()
+    rc = K.lib().kbwt_dev_copy(dst.data_ptr(), src.data_ptr(), nbytes, stream.cuda_stream)
+    if rc != 0 or not torch.equal(src, dst):
+        raise SystemExit("calibration copy failed")
+    log(f"bench: calibration copy of {nbytes} bytes done")
+
+
+def run_single(args, K):
+    import hashlib
+    import torch
+    gen, seed, n, desc = WORKLOADS[args.workload]
     log(f"bench: generating {args.workload} ({n} bytes)")
     T = K.generate(gen, seed, n)
     r = 1 << K.codec18_rlog(n)
@@ -132,84 +180,144 @@ def main():
     dU = torch.empty_like(dT)
     dB = torch.empty_like(dT)
     stream = torch.cuda.current_stream()
-
     log("bench: warmup + parity check")
     I = None
-    for w in range(max(1, args.warmup)):
+    for _ in range(max(1, args.warmup)):
         _, I = K.dev_bwt_aux(dT, r, dU=dU, stream=stream)
         K.dev_unbwt_aux(dU, r, I, dT=dB, stream=stream)
     torch.cuda.synchronize()
     if not torch.equal(dB, dT):
         raise SystemExit("inverse BWT did not restore the block")
     fx = json.load(open(os.path.join(ROOT, "tests", "golden", "fixtures.json")))["inputs"]
-    ref_entry = next((e for e in fx if e["gen"] == gen and e["seed"] == seed and e["n"] == n), None)
+    entry = next((e for e in fx if e["gen"] == gen and e["seed"] == seed and e["n"] == n), None)
     parity = None
-    if ref_entry is not None:
-        import hashlib
-        parity = (hashlib.sha256(dU.cpu().numpy().tobytes()).hexdigest() == ref_entry["sha_U"]
-                  and [int(x) for x in I] == ref_entry["I"])
+    if entry is not None:
+        parity = (hashlib.sha256(dU.cpu().numpy().tobytes()).hexdigest() == entry["sha_U"]
+                  and [int(x) for x in I] == entry["I"])
         if not parity:
             raise SystemExit("BWT differs from kolmo's reference output")
     st = K.last_stats()
     if args.calib_copy:
-        src = torch.empty(args.calib_copy, dtype=torch.uint8, device=dev)
-        dst = torch.empty_like(src)
-        src.fill_(7)
-        torch.cuda.synchronize()
-        rc = K.lib().kbwt_dev_copy(dst.data_ptr(), src.data_ptr(), args.calib_copy, stream.cuda_stream)
-        if rc != 0 or not torch.equal(src, dst):
-            raise SystemExit("calibration copy failed")
-        log(f"bench: calibration copy of {args.calib_copy} bytes done")
+        calib_copy(K, args.calib_copy, dev, stream)
+
+    def step():
+        _, Ix = K.dev_bwt_aux(dT, r, dU=dU, stream=stream)
+        K.dev_unbwt_aux(dU, r, Ix, dT=dB, stream=stream)
 
     log(f"bench: timing {args.steps} steps")
-    L = K.lib()
-    L.kbwt_profile(1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        _, I = K.dev_bwt_aux(dT, r, dU=dU, stream=stream)
-        K.dev_unbwt_aux(dU, r, I, dT=dB, stream=stream)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    recs = (K.KernelTime * 64)()
-    cnt = L.kbwt_profile_read(recs, 64)
-    L.kbwt_profile(0)
-    elapsed = t1 - t0
+    elapsed, kernels = timed_steps(K, args.steps, step)
     ms_step = elapsed / args.steps * 1e3
     value = n * args.steps / elapsed / 2**20
-
-    kernels = {}
-    for i in range(cnt):
-        rec = recs[i]
-        kernels[rec.name.decode()] = {"ms": rec.ms, "launches": rec.launches, "elems": rec.elems}
-    dom = max(kernels, key=lambda k: kernels[k]["ms"])
-    d = kernels[dom]
-    avg_ms = d["ms"] / d["launches"]
-    bytes_per_launch = KERNEL_BYTES.get(dom, 0.0) * d["elems"] / d["launches"]
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = traffic_from_profiles(dom)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
-                "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
-    # whole-path byte model: sum over kernels of algorithmic bytes, over the measured step time
-    path_bytes = sum(KERNEL_BYTES.get(k, 0.0) * v["elems"] for k, v in kernels.items()) / args.steps
-    path = {"bytes_per_step": int(path_bytes), "achieved_GBs": round(path_bytes / (ms_step * 1e-3) / 1e9, 1),
-            "frac": round(path_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "kernel_ms_per_step": {k: round(v["ms"] / args.steps, 4) for k, v in sorted(kernels.items(),
-                                   key=lambda kv: -kv[1]["ms"])}}
+    roofline, path, dom, achieved = roofline_from(kernels, args.steps, ms_step)
     log(f"bench: {value:.1f} MiB/s, {ms_step:.3f} ms/step, dominant {dom} {achieved:.0f} GB/s")
-
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (kolmo's own C on this host)")
-        cpu = cpu_baseline(T, r)
+        cpu = cpu_baseline([T], [r])
+    config = {"workload": f"{args.workload}: {desc}", "n": n, "r": r, "bit_exact_vs_reference": parity,
+              "rounds": st["rounds"], "active": st["active"], "sigma": st["sigma"], "h0": st["h0"]}
+    return value, ms_step, config, roofline, path, cpu
 
+
+def run_batch(args, K):
+    import numpy as np
+    import torch
+    seed, count, lo, hi, kinds, desc = BATCH[args.workload]
+    log(f"bench: generating {args.workload} ({count} blocks)")
+    blocks, kind_of, seeds = K.generate_batch(seed, count, lo, hi, kinds)
+    sizes = [int(b.size) for b in blocks]
+    rs = [1 << K.codec18_rlog(n) for n in sizes]
+    segs, total = K.make_segs(sizes, rs)
+    host = np.concatenate(blocks)
+    dev = torch.device("cuda:0")
+    dT = torch.from_numpy(host).to(dev)
+    dU = torch.empty_like(dT)
+    dB = torch.empty_like(dT)
+    stream = torch.cuda.current_stream()
+    log("bench: warmup + round trip of every block")
+    I = None
+    for _ in range(max(1, args.warmup)):
+        _, I = K.dev_bwt_aux_batch(dT, segs, count, dU=dU, stream=stream)
+        K.dev_unbwt_aux_batch(dU, segs, count, I, dT=dB, stream=stream)
+    torch.cuda.synchronize()
+    if not torch.equal(dB, dT):
+        raise SystemExit("batched inverse BWT did not restore the blocks")
+    # the batched result of a few blocks against the single-block calls
+    U = dU.cpu().numpy()
+    k = 0
+    for b in range(count):
+        ns = K.n_samples(sizes[b], rs[b])
+        if b < 8:
+            u1, i1 = K.bwt_aux(blocks[b], rs[b])
+            o = segs[b].offset
+            if not (np.array_equal(U[o: o + sizes[b]], u1) and np.array_equal(I[k: k + ns], i1)):
+                raise SystemExit(f"batched BWT of block {b} differs from the single-block BWT")
+        k += ns
+    st = K.last_stats()
+
+    def step():
+        _, Ix = K.dev_bwt_aux_batch(dT, segs, count, dU=dU, stream=stream)
+        K.dev_unbwt_aux_batch(dU, segs, count, Ix, dT=dB, stream=stream)
+
+    log(f"bench: timing {args.steps} batched steps")
+    elapsed, kernels = timed_steps(K, args.steps, step)
+    ms_step = elapsed / args.steps * 1e3
+    value = total * args.steps / elapsed / 2**20
+
+    views = [(dT[segs[b].offset: segs[b].offset + sizes[b]], dU[segs[b].offset: segs[b].offset + sizes[b]],
+              dB[segs[b].offset: segs[b].offset + sizes[b]]) for b in range(count)]
+
+    def step_per_block():
+        for b in range(count):
+            t, u, x = views[b]
+            _, Ib = K.dev_bwt_aux(t, rs[b], dU=u, stream=stream)
+            K.dev_unbwt_aux(u, rs[b], Ib, dT=x, stream=stream)
+
+    log("bench: timing one call per block")
+    step_per_block()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step_per_block()
+    torch.cuda.synchronize()
+    per_block_s = time.perf_counter() - t0
+    per_block = total / per_block_s / 2**20
+    roofline, path, dom, achieved = roofline_from(kernels, args.steps, ms_step)
+    log(f"bench: batched {value:.1f} MiB/s, one call per block {per_block:.1f} MiB/s ({value / per_block:.2f}x)")
+    cpu = None
+    if not args.no_cpu:
+        log("bench: CPU baseline (kolmo's own C on this host)")
+        cpu = cpu_baseline(blocks, rs)
+    config = {"workload": f"{args.workload}: {desc}", "blocks": count, "total_bytes": total,
+              "rounds": st["rounds"], "active": st["active"],
+              "one_call_per_block": {"value": round(per_block, 2), "unit": "MiB/s",
+                                     "ms_per_step": round(per_block_s * 1e3, 3)},
+              "batched_over_per_block": round(value / per_block, 3)}
+    return value, ms_step, config, roofline, path, cpu
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="cfg2", choices=sorted(list(WORKLOADS) + list(BATCH)))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="also run one plain device copy of this many bytes (the reference for byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): blocks are independent and a multi-GPU "
+              "run is not part of this build", file=sys.stderr)
+        sys.exit(2)
+    import kolmo_amd as K
+    if not K.device_ok():
+        raise SystemExit("no usable gfx950 device (or libkbwt_hip.so missing)")
+    run = run_batch if args.workload in BATCH else run_single
+    value, ms_step, config, roofline, path, cpu = run(args, K)
     out = {
         "metric": "fwd+inv BWT MiB/s on 1 MI355X", "value": round(value, 2), "unit": "MiB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {desc}", "n": n, "r": r, "bit_exact_vs_reference": parity,
-                   "rounds": st["rounds"], "active": st["active"], "sigma": st["sigma"], "h0": st["h0"]},
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic", "config": config,
         "roofline": roofline, "path_roofline": path, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

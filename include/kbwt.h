@@ -54,10 +54,30 @@ KBWT_API int32_t kolmo_sais_unbwt(const uint8_t *T, uint8_t *U, int32_t *A, int3
 KBWT_API int32_t kbwt_dev_bwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, int32_t *I, void *stream);
 KBWT_API int32_t kbwt_dev_unbwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, const int32_t *I, void *stream);
 
+/* ---- batches (no counterpart in the reference: kolmo calls the single-block entry points once per block) ----
+   count independent blocks in one launch sequence. Block b occupies bytes [offset, offset + n) of T and of U;
+   blocks must not overlap and are given in increasing offset order. r: the block's sampling interval, as for the
+   single-block functions. I holds each block's (n-1)/r + 1 sampled indexes (1 when n <= 1), concatenated in block
+   order. Every block's result is the single-block function's result. count <= 65535; the blocks' total length
+   < 2^31. The dev_ forms take device pointers, like kbwt_dev_bwt_aux. */
+typedef struct {
+    int64_t offset;
+    int32_t n;
+    int32_t r;
+} kbwt_seg;
+KBWT_API int32_t kbwt_bwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs, int32_t count, int32_t *I);
+KBWT_API int32_t kbwt_unbwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs, int32_t count,
+                                      const int32_t *I);
+KBWT_API int32_t kbwt_dev_bwt_aux_batch(const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count,
+                                        int32_t *I, void *stream);
+KBWT_API int32_t kbwt_dev_unbwt_aux_batch(const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count,
+                                          const int32_t *I, void *stream);
+
 /* ---- telemetry of the calling thread's last call ---- */
 #define KBWT_MAX_ROUNDS 40
 typedef struct {
-    int32_t n;                         /* block length */
+    int32_t n;                         /* block length (a batch: the blocks' total length) */
+    int32_t blocks;                    /* blocks of the call */
     int32_t sigma;                     /* distinct byte values */
     int32_t bits_per_symbol;           /* w: bits of the dense symbol code */
     int32_t h0;                        /* symbols in the first sort key */

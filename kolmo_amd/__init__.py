@@ -30,7 +30,8 @@ class KbwtError(RuntimeError):
 
 class Stats(ctypes.Structure):
     _fields_ = [
-        ("n", ctypes.c_int32), ("sigma", ctypes.c_int32), ("bits_per_symbol", ctypes.c_int32),
+        ("n", ctypes.c_int32), ("blocks", ctypes.c_int32), ("sigma", ctypes.c_int32),
+        ("bits_per_symbol", ctypes.c_int32),
         ("h0", ctypes.c_int32), ("rounds", ctypes.c_int32), ("passes", ctypes.c_int32 * MAX_ROUNDS),
         ("active", ctypes.c_int64 * MAX_ROUNDS), ("splitters", ctypes.c_int32),
         ("ms_fwd", ctypes.c_double), ("ms_inv", ctypes.c_double),
@@ -38,7 +39,7 @@ class Stats(ctypes.Structure):
 
     def as_dict(self):
         r = self.rounds
-        return {"n": self.n, "sigma": self.sigma, "bits_per_symbol": self.bits_per_symbol, "h0": self.h0,
+        return {"n": self.n, "blocks": self.blocks, "sigma": self.sigma, "bits_per_symbol": self.bits_per_symbol, "h0": self.h0,
                 "rounds": r, "passes": list(self.passes[: r + 1]), "active": list(self.active[: r + 1]),
                 "splitters": self.splitters, "ms_fwd": self.ms_fwd, "ms_inv": self.ms_inv}
 
@@ -46,6 +47,11 @@ class Stats(ctypes.Structure):
 class KernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("ms", ctypes.c_double), ("launches", ctypes.c_int64),
                 ("elems", ctypes.c_int64)]
+
+
+class Seg(ctypes.Structure):
+    """kbwt_seg: one block of a batch (byte offset in the batch buffers, length, sampling interval)."""
+    _fields_ = [("offset", ctypes.c_int64), ("n", ctypes.c_int32), ("r", ctypes.c_int32)]
 
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
@@ -90,6 +96,15 @@ def lib():
         L.kbwt_profile.argtypes = [ctypes.c_int32]
         L.kbwt_profile_read.restype = ctypes.c_int32
         L.kbwt_profile_read.argtypes = [ctypes.POINTER(KernelTime), ctypes.c_int32]
+        for name in ("kbwt_bwt_aux_batch", "kbwt_unbwt_aux_batch"):
+            f = getattr(L, name)
+            f.restype = ctypes.c_int32
+            f.argtypes = [_u8p, _u8p, ctypes.POINTER(Seg), ctypes.c_int32, _i32p]
+        for name in ("kbwt_dev_bwt_aux_batch", "kbwt_dev_unbwt_aux_batch"):
+            f = getattr(L, name)
+            f.restype = ctypes.c_int32
+            f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Seg), ctypes.c_int32, _i32p,
+                          ctypes.c_void_p]
         L.kbwt_dev_copy.restype = ctypes.c_int32
         L.kbwt_dev_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
         _lib = L
@@ -163,6 +178,70 @@ def dev_unbwt_aux(dU, r, I, dT=None, stream=None):
     return dT
 
 
+def make_segs(sizes, rs, gaps=None):
+    """Segments for blocks of the given sizes packed back to back (optionally with gap bytes after each)."""
+    segs = (Seg * max(len(sizes), 1))()
+    at = 0
+    for b, (n, r) in enumerate(zip(sizes, rs)):
+        segs[b].offset, segs[b].n, segs[b].r = at, int(n), int(r)
+        at += int(n) + (int(gaps[b]) if gaps is not None else 0)
+    return segs, at
+
+
+def samples_total(sizes, rs):
+    return sum(n_samples(int(n), int(r)) for n, r in zip(sizes, rs))
+
+
+def bwt_aux_batch(blocks, rs):
+    """Forward BWT of a list of blocks (uint8 arrays) in one batch: returns (list of U, list of I)."""
+    sizes = [int(b.size) for b in blocks]
+    segs, total = make_segs(sizes, rs)
+    T = np.concatenate([np.asarray(b, np.uint8) for b in blocks]) if blocks else np.empty(0, np.uint8)
+    T = np.ascontiguousarray(np.concatenate([T, np.zeros(1, np.uint8)]))
+    U = np.empty_like(T)
+    I = np.zeros(max(samples_total(sizes, rs), 1), np.int32)
+    _check(lib().kbwt_bwt_aux_batch(_ptr(T), _ptr(U), segs, len(blocks), _ptr(I, _i32p)))
+    outU, outI, k = [], [], 0
+    for b, n in enumerate(sizes):
+        outU.append(U[segs[b].offset: segs[b].offset + n].copy())
+        ns = n_samples(n, rs[b])
+        outI.append(I[k: k + ns].copy())
+        k += ns
+    return outU, outI
+
+
+def unbwt_aux_batch(blocks, rs, Is):
+    sizes = [int(b.size) for b in blocks]
+    segs, total = make_segs(sizes, rs)
+    T = np.concatenate([np.asarray(b, np.uint8) for b in blocks]) if blocks else np.empty(0, np.uint8)
+    T = np.ascontiguousarray(np.concatenate([T, np.zeros(1, np.uint8)]))
+    U = np.empty_like(T)
+    I = np.ascontiguousarray(np.concatenate(list(Is)) if Is else np.zeros(1, np.int32), np.int32)
+    _check(lib().kbwt_unbwt_aux_batch(_ptr(T), _ptr(U), segs, len(blocks), _ptr(I, _i32p)))
+    return [U[segs[b].offset: segs[b].offset + n].copy() for b, n in enumerate(sizes)]
+
+
+def dev_bwt_aux_batch(dT, segs, count, dU=None, stream=None):
+    """Device-resident batch forward over a uint8 tensor holding the blocks at segs' offsets: returns (dU, I)."""
+    import torch
+    if dU is None:
+        dU = torch.empty_like(dT)
+    I = np.zeros(max(sum(n_samples(segs[b].n, segs[b].r) for b in range(count)), 1), np.int32)
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _check(lib().kbwt_dev_bwt_aux_batch(dT.data_ptr(), dU.data_ptr(), segs, count, _ptr(I, _i32p), s))
+    return dU, I
+
+
+def dev_unbwt_aux_batch(dU, segs, count, I, dT=None, stream=None):
+    import torch
+    if dT is None:
+        dT = torch.empty_like(dU)
+    I = np.ascontiguousarray(I, np.int32)
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _check(lib().kbwt_dev_unbwt_aux_batch(dU.data_ptr(), dT.data_ptr(), segs, count, _ptr(I, _i32p), s))
+    return dT
+
+
 def last_stats():
     st = Stats()
     lib().kbwt_last_stats(ctypes.byref(st))
@@ -199,13 +278,15 @@ GEN_KINDS = {"markov2": 0, "mixed": 1, "repeats": 2, "dna": 3, "random": 4, "csv
              "text": 8}
 
 
-def generate(kind, seed, n):
-    """Seeded synthetic block (see kolmo_amd/csrc/kbwt_gen.c for each generator)."""
+def generate(kind, seed, n, model_seed=None):
+    """Seeded synthetic block (see kolmo_amd/csrc/kbwt_gen.c for each generator). model_seed (default: seed) draws
+    the Markov models, so blocks of one batch can share their source statistics."""
     g = gen_lib()
-    g.kbwt_gen_kind.restype = ctypes.c_int
-    g.kbwt_gen_kind.argtypes = [ctypes.c_int, ctypes.c_uint64, _u8p, ctypes.c_size_t]
+    g.kbwt_gen_kind2.restype = ctypes.c_int
+    g.kbwt_gen_kind2.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, _u8p, ctypes.c_size_t]
     out = np.empty(max(n, 1), np.uint8)
-    rc = g.kbwt_gen_kind(GEN_KINDS[kind] if isinstance(kind, str) else int(kind), seed, _ptr(out), n)
+    rc = g.kbwt_gen_kind2(GEN_KINDS[kind] if isinstance(kind, str) else int(kind),
+                          seed if model_seed is None else model_seed, seed, _ptr(out), n)
     if rc != 0:
         raise RuntimeError(f"generator {kind} failed ({rc})")
     return out[:n]
@@ -235,6 +316,17 @@ def batch_plan(seed, count, lo, hi, kinds):
     if rc != 0:
         raise ValueError("bad batch plan")
     return sizes, kout, seeds
+
+
+def generate_batch(seed, count, lo, hi, kinds, threads=8):
+    """A batch of seeded blocks: sizes log-uniform in [lo, hi], each from one of `kinds` (Markov models drawn from
+    the batch seed). Returns (list of uint8 arrays, kinds, seeds)."""
+    from concurrent.futures import ThreadPoolExecutor
+    sizes, kind_of, seeds = batch_plan(seed, count, lo, hi, kinds)
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        blocks = list(ex.map(lambda b: generate(int(kind_of[b]), int(seeds[b]), int(sizes[b]), model_seed=seed),
+                             range(count)))
+    return blocks, kind_of, seeds
 
 
 # The measured workloads: (generator, seed, n).

@@ -46,6 +46,82 @@ __device__ __forceinline__ uint32_t lookback(const uint64_t *status, size_t stri
     return acc;
 }
 
+// The same with a window: one thread reads the status words of LB earlier tiles per round trip, adds the ready
+// aggregates from the nearest one on, stops at the first inclusive prefix and re-reads from the first word that was
+// not ready yet.
+template <int LB>
+__device__ __forceinline__ uint32_t lookback_win(const uint64_t *status, size_t stride, uint32_t tile,
+                                                 uint32_t epoch) {
+    uint32_t acc = 0;
+    int64_t t = (int64_t)tile - 1;
+    const uint32_t want = epoch & EPOCH_MASK;
+    while (t >= 0) {
+        uint64_t w[LB];
+#pragma unroll
+        for (int j = 0; j < LB; j++) w[j] = (t - j >= 0) ? st_load(status + (size_t)(t - j) * stride) : 0ull;
+        uint32_t add = 0;
+        int adv = 0;
+        bool stop = false, done = false;
+#pragma unroll
+        for (int j = 0; j < LB; j++) {
+            if (!stop) {
+                if (t - j < 0) {
+                    stop = true;
+                    done = true;
+                } else {
+                    uint32_t e = (uint32_t)(w[j] >> 34), f = (uint32_t)(w[j] >> 32) & 3u;
+                    if (e != want || f == 0) stop = true;
+                    else {
+                        add += (uint32_t)w[j];
+                        adv = j + 1;
+                        if (f == ST_INC) { stop = true; done = true; }
+                    }
+                }
+            }
+        }
+        acc += add;
+        t -= adv;
+        if (done) break;
+        if (adv == 0) __builtin_amdgcn_s_sleep(1);
+    }
+    return acc;
+}
+
+// Wave-wide lookback of one quantity (sum or max): lane j reads the word of tile t-1-j, so one round trip covers 64
+// earlier tiles. Every lane of the wave must call it; all get the result.
+template <bool MAX>
+__device__ __forceinline__ uint32_t lookback_wave(const uint64_t *status, uint32_t tile, uint32_t epoch) {
+    const uint32_t lane = __lane_id();
+    const uint32_t want = epoch & EPOCH_MASK;
+    uint32_t acc = 0;
+    int64_t t = (int64_t)tile - 1;
+    while (t >= 0) {
+        int64_t mine = t - (int64_t)lane;
+        uint64_t w = mine >= 0 ? st_load(status + mine) : 0ull;
+        uint32_t e = (uint32_t)(w >> 34), f = (uint32_t)(w >> 32) & 3u;
+        bool ready = mine >= 0 && e == want && f != 0;
+        bool inc = ready && f == ST_INC;
+        uint64_t notready = ~__ballot(ready);
+        uint64_t incs = __ballot(inc);
+        // lanes that count: those before the first lane that is not ready, and up to the first inclusive one
+        uint32_t first_nr = notready ? (uint32_t)__builtin_ctzll(notready) : 64u;
+        uint32_t first_inc = incs ? (uint32_t)__builtin_ctzll(incs) : 64u;
+        bool done = first_inc < first_nr;
+        uint32_t upto = done ? first_inc + 1 : first_nr;
+        uint32_t v = lane < upto ? (uint32_t)w : 0u;
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) {
+            uint32_t u = __shfl_xor(v, o, WAVE);
+            v = MAX ? (v > u ? v : u) : v + u;
+        }
+        acc = MAX ? (acc > v ? acc : v) : acc + v;
+        if (done) break;
+        if (upto == 0) __builtin_amdgcn_s_sleep(1);
+        t -= upto;
+    }
+    return acc;
+}
+
 // Tile id from a per-launch counter; the workgroup that draws the last id resets it for the next launch (all
 // ntiles draws have happened by then).
 __device__ __forceinline__ uint32_t next_tile(uint32_t *counter, uint32_t ntiles) {

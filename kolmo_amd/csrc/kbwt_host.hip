@@ -1,6 +1,6 @@
-// Host side of libkbwt_hip: the C-ABI entry points of include/kbwt.h, the per-device workspace, and the round loop
-// of the forward suffix sort and the steps of the inverse. Validation and return codes follow
-// src/algo/sais/sais.c:12027-12082 of kolmo 0.0.2.
+// Host side of libkbwt_hip: the C-ABI entry points of include/kbwt.h, the per-device workspace, and the steps of the
+// forward suffix sort (round loop) and of the inverse, for a batch of blocks (one block is the batch of one).
+// Validation and return codes follow src/algo/sais/sais.c:12027-12082 of kolmo 0.0.2.
 #include "kbwt_internal.h"
 #include "../../include/kbwt.h"
 
@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -23,7 +24,7 @@ struct DevError {
     int code;
 };
 
-void fail(hipError_t e, const char *what) {
+[[noreturn]] void fail(hipError_t e, const char *what) {
     char buf[256];
     snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
     g_last_error = buf;
@@ -35,7 +36,7 @@ void fail(hipError_t e, const char *what) {
         hipError_t e_ = (x);                           \
         if (e_ != hipSuccess) fail(e_, #x);            \
     } while (0)
-#define LAUNCHCHK(what) HIPCHK(hipGetLastError())
+#define LAUNCHCHK() HIPCHK(hipGetLastError())
 
 struct Buf {
     void *p = nullptr;
@@ -47,27 +48,26 @@ struct Workspace {
     hipStream_t stream = nullptr;
     std::mutex mu;
     uint32_t epoch = 1;
-    Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, st_max, samples, stage;
-    // small fixed device buffers
-    uint32_t *counters = nullptr;      // 16 tile counters (zeroed once)
-    uint32_t *counts = nullptr;        // MAXP * 256 histogram accumulators, zero between uses
-    uint32_t *tot = nullptr;           // 256 byte counts
-    uint32_t *dbase = nullptr;         // MAXP * 256
-    uint32_t *cbase = nullptr;         // 256
-    uint8_t *code = nullptr;           // 256
-    unsigned long long *orand = nullptr;  // 2
-    uint32_t *m_out = nullptr;         // 1
-    // pinned host mirror
-    uint32_t *h_counts = nullptr;      // 256
-    uint8_t *h_code = nullptr;         // 256
-    unsigned long long *h_orand = nullptr;  // 2 (also staging for the reset value)
-    unsigned long long *h_orand_init = nullptr;
+    Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, st_max, samples, stage, blk, counts, dbase,
+        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, lo;
+    uint32_t *counters = nullptr;           // 16 tile counters (zeroed once)
+    uint32_t *gcnt = nullptr;               // 4 group-list counters
+    uint32_t *h_gcnt = nullptr;
+    unsigned long long *orand = nullptr;    // 2
+    uint32_t *m_out = nullptr;              // 1
+    // pinned host staging
+    void *h_meta = nullptr;
+    size_t h_meta_cap = 0;
+    unsigned long long *h_orand = nullptr, *h_orand_init = nullptr;
     uint32_t *h_m = nullptr;
-    uint64_t *h_link = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool ready = false;
     // per-kernel device time (HIP events around each launch, on the launch's stream) while profiling is on
-    struct Pending { int slot; hipEvent_t a, b; uint64_t elems; };
+    struct Pending {
+        int slot;
+        hipEvent_t a, b;
+        uint64_t elems;
+    };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> spare;
 };
@@ -83,7 +83,8 @@ bool g_prof_on = false;
 std::vector<KernelTime> g_prof;
 
 int prof_slot(const char *name) {
-    for (size_t i = 0; i < g_prof.size(); i++) if (g_prof[i].name == name) return (int)i;
+    for (size_t i = 0; i < g_prof.size(); i++)
+        if (g_prof[i].name == name) return (int)i;
     g_prof.push_back(KernelTime{name});
     return (int)g_prof.size() - 1;
 }
@@ -106,6 +107,18 @@ void grow(Buf &b, size_t bytes, bool zero = false) {
     b.cap = cap;
 }
 
+void *host_staging(Workspace &w, size_t bytes) {
+    if (w.h_meta_cap < bytes) {
+        if (w.h_meta) HIPCHK(hipHostFree(w.h_meta));
+        w.h_meta = nullptr;
+        w.h_meta_cap = 0;
+        size_t cap = bytes + bytes / 4 + 4096;
+        HIPCHK(hipHostMalloc(&w.h_meta, cap, hipHostMallocDefault));
+        w.h_meta_cap = cap;
+    }
+    return w.h_meta;
+}
+
 Workspace &workspace() {
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
@@ -118,21 +131,13 @@ Workspace &workspace() {
         HIPCHK(hipMalloc(&w.counters, 16 * sizeof(uint32_t)));
         HIPCHK(hipMemset(w.counters, 0, 16 * sizeof(uint32_t)));
         HIPCHK(hipDeviceSynchronize());
-        HIPCHK(hipMalloc(&w.counts, MAXP * 256 * sizeof(uint32_t)));
-        HIPCHK(hipMemset(w.counts, 0, MAXP * 256 * sizeof(uint32_t)));
-        HIPCHK(hipDeviceSynchronize());
-        HIPCHK(hipMalloc(&w.tot, 256 * sizeof(uint32_t)));
-        HIPCHK(hipMalloc(&w.dbase, MAXP * 256 * sizeof(uint32_t)));
-        HIPCHK(hipMalloc(&w.cbase, 256 * sizeof(uint32_t)));
-        HIPCHK(hipMalloc(&w.code, 256));
         HIPCHK(hipMalloc(&w.orand, 2 * sizeof(unsigned long long)));
         HIPCHK(hipMalloc(&w.m_out, sizeof(uint32_t)));
-        HIPCHK(hipHostMalloc(&w.h_counts, 256 * sizeof(uint32_t), hipHostMallocDefault));
-        HIPCHK(hipHostMalloc(&w.h_code, 256, hipHostMallocDefault));
+        HIPCHK(hipMalloc(&w.gcnt, 4 * sizeof(uint32_t)));
+        HIPCHK(hipHostMalloc(&w.h_gcnt, 4 * sizeof(uint32_t), hipHostMallocDefault));
         HIPCHK(hipHostMalloc(&w.h_orand, 2 * sizeof(unsigned long long), hipHostMallocDefault));
         HIPCHK(hipHostMalloc(&w.h_orand_init, 2 * sizeof(unsigned long long), hipHostMallocDefault));
         HIPCHK(hipHostMalloc(&w.h_m, sizeof(uint32_t), hipHostMallocDefault));
-        HIPCHK(hipHostMalloc(&w.h_link, sizeof(uint64_t), hipHostMallocDefault));
         w.h_orand_init[0] = 0ull;
         w.h_orand_init[1] = ~0ull;
         HIPCHK(hipEventCreate(&w.ev0));
@@ -151,10 +156,20 @@ template <class F> void timed(Workspace &w, hipStream_t s, const char *name, uin
         on = g_prof_on;
         if (on) slot = prof_slot(name);
     }
-    if (!on) { fn(); return; }
+    if (!on) {
+        fn();
+        return;
+    }
     hipEvent_t a, b;
-    if (w.spare.size() >= 2) { a = w.spare.back(); w.spare.pop_back(); b = w.spare.back(); w.spare.pop_back(); }
-    else { HIPCHK(hipEventCreate(&a)); HIPCHK(hipEventCreate(&b)); }
+    if (w.spare.size() >= 2) {
+        a = w.spare.back();
+        w.spare.pop_back();
+        b = w.spare.back();
+        w.spare.pop_back();
+    } else {
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+    }
     HIPCHK(hipEventRecord(a, s));
     fn();
     HIPCHK(hipEventRecord(b, s));
@@ -177,6 +192,7 @@ void prof_collect(Workspace &w) {
     }
     w.pending.clear();
 }
+#define TIMED(name, elems, call) timed(w, s, name, (uint64_t)(elems), [&] { call; })
 
 inline uint32_t ceil_log2(uint32_t x) {
     uint32_t b = 0;
@@ -185,10 +201,6 @@ inline uint32_t ceil_log2(uint32_t x) {
 }
 template <class T> T *P(Buf &b) { return reinterpret_cast<T *>(b.p); }
 
-uint32_t hist_grid(uint32_t n) { return std::max(1u, std::min<uint32_t>(HIST_GRID, (n + 8191) / 8192)); }
-
-#define TIMED(name, elems, call) timed(w, s, name, (uint64_t)(elems), [&] { call; })
-
 uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi) {
     const char *v = getenv(name);
     if (!v || !*v) return dflt;
@@ -196,167 +208,341 @@ uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi) {
     return x < (long)lo ? lo : (x > (long)hi ? hi : (uint32_t)x);
 }
 
-// Clears the histogram accumulators: they are left at zero by every scan, and this makes a call that follows an
-// interrupted one start clean too.
-void reset_counts(Workspace &w, hipStream_t s) { launch_fill_u8(reinterpret_cast<uint8_t *>(w.counts), 0, MAXP * 256 * 4, s); }
+// first[b] = tiles of the blocks before b, first[B] = all tiles.
+template <class F> uint32_t fill_tiles(uint32_t *first, uint32_t B, F tiles_of) {
+    uint32_t t = 0;
+    for (uint32_t b = 0; b < B; b++) {
+        first[b] = t;
+        t += tiles_of(b);
+    }
+    first[B] = t;
+    return t;
+}
+inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // ------------------------------------------------------------------ forward
 
-void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t r, int32_t *I, hipStream_t s) {
+// bl: one BlockInfo per block with off, n (>= 2), r and soff filled in; packed input dT (N bytes), output dU.
+// I: host array of nsamp sampled indexes. Per-call metadata (blocks, tile tables, code tables) goes to the device
+// in one copy.
+void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo> &bl, uint32_t N, int32_t *I,
+             uint32_t nsamp, hipStream_t s) {
+    const uint32_t B = (uint32_t)bl.size();
     kbwt_stats &st = g_stats;
     memset(&st, 0, sizeof st);
-    st.n = (int32_t)n;
-    const size_t N = std::max<size_t>(n, 64);
-    grow(w.K0, N * 8);
-    grow(w.K1, N * 8);
-    grow(w.V0, N * 4);
-    grow(w.V1, N * 4);
-    grow(w.SA, N * 4);
-    grow(w.ISA, N * 4);
-    grow(w.L, N);
-    for (int i = 0; i < 2; i++) { grow(w.pos[i], N * 4); grow(w.grp[i], N * 4); }
-    const uint32_t ntiles_max = (uint32_t)((N + std::min(SORT_TILE, RG_TILE) - 1) / std::min(SORT_TILE, RG_TILE));
-    grow(w.status, (size_t)ntiles_max * 256 * 8, true);
-    grow(w.st_sum, (size_t)ntiles_max * 8, true);
-    grow(w.st_max, (size_t)ntiles_max * 8, true);
-    const uint32_t nsamp = (n - 1) / r + 1;
+    st.n = (int32_t)N;
+    st.blocks = (int32_t)B;
+    const int scfg = (int)env_u32("KBWT_SORT_CFG", 0, 0, SORT_NCFG - 1);
+    const uint32_t stile = sort_tile(scfg);
+    const size_t NN = std::max<size_t>(N, 64);
+    grow(w.K0, NN * 8);
+    grow(w.K1, NN * 8);
+    grow(w.V0, NN * 4);
+    grow(w.V1, NN * 4);
+    grow(w.SA, NN * 4);
+    grow(w.ISA, NN * 4);
+    grow(w.L, NN);
+    for (int i = 0; i < 2; i++) {
+        grow(w.pos[i], NN * 4);
+        grow(w.grp[i], NN * 4);
+    }
+    const size_t ntiles_max = (NN + MIN_TILE - 1) / MIN_TILE + B;
+    grow(w.status, ntiles_max * 256 * 8, true);
+    grow(w.st_sum, ntiles_max * 8, true);
+    grow(w.st_max, ntiles_max * 8, true);
     grow(w.samples, (size_t)nsamp * 4);
+    grow(w.counts, (size_t)B * MAXP * 256 * 4, true);
+    grow(w.tot, (size_t)B * 256 * 4);
+    grow(w.dbase, (size_t)B * MAXP * 256 * 4);
+    if (B > 1) grow(w.blk, NN * 2);
+
+    // metadata layout: blocks | hist tiles | sort tiles | codes; the byte counts come back behind it
+    const size_t o_ht = align16(B * sizeof(BlockInfo)), o_st = o_ht + (B + 1) * 4, o_codes = align16(o_st + (B + 1) * 4);
+    const size_t bytes = o_codes + (size_t)B * 256;
+    uint8_t *hm = (uint8_t *)host_staging(w, bytes + (size_t)B * 256 * 4);
+    grow(w.meta, bytes);
+    uint8_t *dm = P<uint8_t>(w.meta);
+    BlockInfo *h_bl = (BlockInfo *)hm;
+    uint8_t *h_codes = hm + o_codes;
+    uint32_t *h_counts = (uint32_t *)(hm + bytes);
+    const BlockInfo *d_bl = (const BlockInfo *)dm;
+    const uint8_t *d_codes = dm + o_codes;
+    const uint32_t nht = fill_tiles((uint32_t *)(hm + o_ht), B, [&](uint32_t b) { return (bl[b].n + HIST_TILE - 1) / HIST_TILE; });
+    const uint32_t nst = fill_tiles((uint32_t *)(hm + o_st), B, [&](uint32_t b) { return (bl[b].n + stile - 1) / stile; });
+    memcpy(h_bl, bl.data(), B * sizeof(BlockInfo));
+    HIPCHK(hipMemcpyAsync(dm, hm, o_codes, hipMemcpyHostToDevice, s));
+    const Tiling thist{(const uint32_t *)(dm + o_ht), d_bl, B}, tsort{(const uint32_t *)(dm + o_st), d_bl, B};
+    const uint16_t *blk = B > 1 ? P<uint16_t>(w.blk) : nullptr;
 
     HIPCHK(hipEventRecord(w.ev0, s));
-    reset_counts(w, s);
-    // 1. byte histogram -> dense symbol code
-    const uint32_t g1 = hist_grid(n);
-    TIMED("byte_hist", n, launch_byte_hist(dT, n, w.counts, g1, s));
-    TIMED("scan_digits", 256, launch_scan_digits(w.counts, 1, w.tot, nullptr, 0, s));
+    // histogram accumulators start at zero (every scan clears them; this covers a call that stopped half way)
+    launch_fill_u8(P<uint8_t>(w.counts), 0, (size_t)B * MAXP * 256 * 4, s);
+    if (B > 1) TIMED("fill_blk", N, launch_fill_blk(P<uint16_t>(w.blk), d_bl, B, N, s));
+    // 1. byte histograms -> dense symbol codes per block
+    TIMED("byte_hist", N, launch_byte_hist(dT, thist, nht, P<uint32_t>(w.counts), s));
+    TIMED("scan_digits", 256 * B,
+          launch_scan_digits(P<uint32_t>(w.counts), B, 1, d_bl, 0, P<uint32_t>(w.tot), nullptr, s));
     LAUNCHCHK();
-    HIPCHK(hipMemcpyAsync(w.h_counts, w.tot, 256 * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h_counts, w.tot.p, (size_t)B * 256 * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    uint32_t sigma = 0;
-    for (int c = 0; c < 256; c++) { w.h_code[c] = (uint8_t)sigma; if (w.h_counts[c]) sigma++; }
-    for (int c = 0; c < 256; c++) if (!w.h_counts[c]) w.h_code[c] = 0;
     // 56 key bits above the BWT byte: 8 symbols when a symbol needs 5-7 bits, 7 bytes of binary data, and as many
     // symbols as fit for small alphabets (14 of 4 bits, 18 of 3, 28 of 2, 56 of 1)
-    const uint32_t wbits = sigma <= 2 ? 1 : ceil_log2(sigma);
-    const uint32_t h0 = wbits >= 5 ? std::min<uint32_t>(8, 56 / wbits) : 56 / wbits;
-    const uint32_t P0 = (h0 * wbits + 7) / 8;
-    st.sigma = (int32_t)sigma;
-    st.bits_per_symbol = (int32_t)wbits;
-    st.h0 = (int32_t)h0;
-    HIPCHK(hipMemcpyAsync(w.code, w.h_code, 256, hipMemcpyHostToDevice, s));
+    uint32_t P0 = 1;
+    for (uint32_t b = 0; b < B; b++) {
+        const uint32_t *cnt = h_counts + (size_t)b * 256;
+        uint8_t *code = h_codes + (size_t)b * 256;
+        uint32_t sigma = 0;
+        for (int c = 0; c < 256; c++) {
+            code[c] = cnt[c] ? (uint8_t)sigma : 0;
+            if (cnt[c]) sigma++;
+        }
+        const uint32_t wb = sigma <= 2 ? 1 : ceil_log2(sigma);
+        bl[b].w = wb;
+        bl[b].h0 = wb >= 5 ? std::min<uint32_t>(8, 56 / wb) : 56 / wb;
+        P0 = std::max(P0, (bl[b].h0 * wb + 7) / 8);
+        if (b == 0) {
+            st.sigma = (int32_t)sigma;
+            st.bits_per_symbol = (int32_t)wb;
+            st.h0 = (int32_t)bl[b].h0;
+        }
+    }
+    memcpy(h_bl, bl.data(), B * sizeof(BlockInfo));
+    HIPCHK(hipMemcpyAsync(dm, hm, bytes, hipMemcpyHostToDevice, s));
 
     // 2. digit histograms of the first keys
-    TIMED("hist_T", n, launch_hist_T(dT, n, w.code, wbits, h0, P0, w.counts, g1, s));
-    TIMED("scan_digits", 256 * P0, launch_scan_digits(w.counts, P0, nullptr, w.dbase, 0, s));
+    TIMED("hist_T", N, launch_hist_T(dT, thist, nht, d_codes, P0, P<uint32_t>(w.counts), s));
+    TIMED("scan_digits", 256 * P0 * B,
+          launch_scan_digits(P<uint32_t>(w.counts), B * P0, P0, d_bl, 1, nullptr, P<uint32_t>(w.dbase), s));
     LAUNCHCHK();
 
-    // 3. the first sort: P0 stable passes over the digits above the BWT byte, the last one writing the suffix array
+    // 3. the first sort: P0 stable passes per block over the digits above the BWT byte; the last writes the SA
     uint64_t *K[2] = {P<uint64_t>(w.K0), P<uint64_t>(w.K1)};
     uint32_t *V[2] = {P<uint32_t>(w.V0), P<uint32_t>(w.V1)};
     uint32_t *SA = P<uint32_t>(w.SA), *ISA = P<uint32_t>(w.ISA);
     uint8_t *L = P<uint8_t>(w.L);
     for (uint32_t p = 0; p < P0; p++) {
         SortArgs a{};
+        a.cfg = scfg;
         a.from_t = p == 0;
         a.kin = p ? K[(p - 1) & 1] : nullptr;
         a.vin = p ? V[(p - 1) & 1] : nullptr;
         a.kout = K[p & 1];
         a.vout = p == P0 - 1 ? SA : V[p & 1];
-        a.m = n;
+        a.m = N;
         a.shift = 8 * p + 8;
-        a.dbase = w.dbase + p * 256;
+        a.dbase = P<uint32_t>(w.dbase) + p * 256;
+        a.dstride = P0 * 256;
+        a.tiling = tsort;
         a.status = P<uint64_t>(w.status);
         a.counter = w.counters + 0;
         a.epoch = w.epoch++;
         a.T = dT;
-        a.code = w.code;
-        a.w = wbits;
-        a.h0 = h0;
-        TIMED(p == 0 ? "sort_pass_T" : "sort_pass", n, launch_sort_pass(a, s));
+        a.codes = d_codes;
+        TIMED(p == 0 ? "sort_pass_T" : "sort_pass", N, launch_sort_pass(a, nst, s));
         LAUNCHCHK();
     }
     st.passes[0] = (int32_t)P0;
-    st.active[0] = n;
+    st.active[0] = N;
 
     // 4. groups after the first sort; the BWT byte of every row is its key's low byte
     {
         RegroupArgs g{};
-        g.K = K[(P0 - 1) & 1]; g.V = SA; g.pos = nullptr; g.m = n; g.kshift = 8;
-        g.SA = nullptr; g.ISA = ISA; g.L = L; g.T = nullptr;
-        g.pos_out = P<uint32_t>(w.pos[0]); g.grp_out = P<uint32_t>(w.grp[0]); g.m_out = w.m_out;
-        g.st_sum = P<uint64_t>(w.st_sum); g.st_max = P<uint64_t>(w.st_max); g.counter = w.counters + 1;
+        g.K = K[(P0 - 1) & 1];
+        g.V = SA;
+        g.pos = nullptr;
+        g.m = N;
+        g.kshift = 8;
+        g.blk = blk;
+        g.SA = nullptr;
+        g.ISA = ISA;
+        g.L = L;
+        g.T = nullptr;
+        g.pos_out = P<uint32_t>(w.pos[0]);
+        g.grp_out = P<uint32_t>(w.grp[0]);
+        g.m_out = w.m_out;
+        g.st_sum = P<uint64_t>(w.st_sum);
+        g.st_max = P<uint64_t>(w.st_max);
+        g.counter = w.counters + 1;
         g.epoch = w.epoch++;
-        TIMED("regroup", n, launch_regroup(g, s));
+        TIMED("regroup", N, launch_regroup(g, s));
     }
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     uint32_t m = *w.h_m;
 
-    // 5. doubling rounds over the rows still in groups of two or more
-    uint64_t h = h0;
+    // 5. doubling rounds over the groups of two or more (SA ranges). Small groups sort inside one workgroup each
+    // (k_seg_round); groups of more than SEG_TILE rows take the global path (round keys, radix passes, regroup).
+    uint32_t hmin = 56;
+    for (uint32_t b = 0; b < B; b++) hmin = std::min(hmin, bl[b].h0);
+    uint64_t h = hmin;
+    const size_t GMAX = NN / 2 + 2;   // groups have two rows or more
+    for (int i = 0; i < 2; i++) {
+        grow(w.gh[i], GMAX * 4);
+        grow(w.gs[i], GMAX * 4);
+        grow(w.lh[i], GMAX * 4);
+        grow(w.ls[i], GMAX * 4);
+    }
+    grow(w.go, GMAX * 4);
+    grow(w.sec, NN * 4);
+    grow(w.lo, GMAX * 8);
+    uint32_t *gcnt = w.gcnt;          // [small groups, large groups, small rows] of the list being built
+    std::vector<uint32_t> h_lh, h_ls, h_lo;
+    auto read_lists = [&](int c, uint32_t &Gs, uint32_t &Gl) {
+        HIPCHK(hipMemcpyAsync(w.h_gcnt, gcnt, 16, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        Gs = w.h_gcnt[0];
+        Gl = w.h_gcnt[1];
+        h_lh.resize(Gl);
+        h_ls.resize(Gl);
+        if (Gl) {
+            HIPCHK(hipMemcpyAsync(h_lh.data(), w.lh[c].p, (size_t)Gl * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(h_ls.data(), w.ls[c].p, (size_t)Gl * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
+    };
+    auto zero_counts = [&] { launch_fill_u8(reinterpret_cast<uint8_t *>(gcnt), 0, 16, s); };
     int cur = 0, round = 0;
-    while (m > 0) {
+    uint32_t Gs = 0, Gl = 0;
+    zero_counts();
+    if (m) {
+        TIMED("to_groups", m,
+              launch_elems_to_groups(P<uint32_t>(w.pos[0]), P<uint32_t>(w.grp[0]), w.m_out, m, P<uint32_t>(w.gh[0]),
+                                     P<uint32_t>(w.gs[0]), P<uint32_t>(w.lh[0]), P<uint32_t>(w.ls[0]), gcnt, s));
+        LAUNCHCHK();
+    }
+    read_lists(0, Gs, Gl);
+    const Tiling flat{nullptr, d_bl, 1};
+    while (Gs + Gl > 0) {
         round++;
         if (round >= KBWT_MAX_ROUNDS) fail(hipErrorUnknown, "doubling did not converge");
-        const uint32_t hh = (uint32_t)std::min<uint64_t>(h, n);
-        HIPCHK(hipMemcpyAsync(w.orand, w.h_orand_init, 16, hipMemcpyHostToDevice, s));
-        TIMED("round_keys", m,
-              launch_round_keys(P<uint32_t>(w.pos[cur]), P<uint32_t>(w.grp[cur]), m, SA, ISA, n, hh, K[0], V[0],
-                                w.orand, s));
-        LAUNCHCHK();
-        HIPCHK(hipMemcpyAsync(w.h_orand, w.orand, 16, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        const uint64_t vary = w.h_orand[0] ^ w.h_orand[1];
-        Shifts sh{};
-        for (uint32_t d = 0; d < 8; d++)
-            if ((vary >> (8 * d)) & 0xFFull) sh.s[sh.P++] = 8 * d;
-        int src = 0;
-        if (sh.P) {
-            const uint32_t g2 = hist_grid(m);
-            TIMED("hist_keys", m, launch_hist_keys(K[0], m, sh, w.counts, g2, s));
-            TIMED("scan_digits", 256 * sh.P, launch_scan_digits(w.counts, sh.P, nullptr, w.dbase, 0, s));
+        const uint32_t hh = (uint32_t)std::min<uint64_t>(h, N);
+        uint32_t Es = 0, ml = 0;
+        // gather every key of the round before any rank changes
+        if (Gs) {
+            TIMED("scan_groups", Gs,
+                  launch_scan_groups(P<uint32_t>(w.gs[cur]), Gs, P<uint32_t>(w.go), w.m_out, P<uint64_t>(w.st_sum),
+                                     w.counters + 3, w.epoch++, s));
             LAUNCHCHK();
-            for (uint32_t p = 0; p < sh.P; p++) {
-                SortArgs a{};
-                a.from_t = false;
-                a.kin = K[src];
-                a.vin = V[src];
-                a.kout = K[src ^ 1];
-                a.vout = V[src ^ 1];
-                a.m = m;
-                a.shift = sh.s[p];
-                a.dbase = w.dbase + p * 256;
-                a.status = P<uint64_t>(w.status);
-                a.counter = w.counters + 0;
-                a.epoch = w.epoch++;
-                TIMED("sort_pass", m, launch_sort_pass(a, s));
-                LAUNCHCHK();
-                src ^= 1;
+            HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            Es = *w.h_m;
+        }
+        SegRoundArgs sa{};
+        sa.gh = P<uint32_t>(w.gh[cur]);
+        sa.go = P<uint32_t>(w.go);
+        sa.G = Gs;
+        sa.E = Es;
+        sa.SA = SA;
+        sa.ISA = ISA;
+        sa.sec = P<uint32_t>(w.sec);
+        sa.L = L;
+        sa.T = dT;
+        sa.blk = blk;
+        sa.blocks = d_bl;
+        sa.n = N;
+        sa.h = hh;
+        sa.nh = P<uint32_t>(w.gh[cur ^ 1]);
+        sa.ns = P<uint32_t>(w.gs[cur ^ 1]);
+        sa.ncnt = gcnt;
+        if (Gs) TIMED("seg_gather", Es, launch_seg_round(0, sa, s));
+        Shifts sh{};
+        if (Gl) {
+            // the large groups in increasing head order (regroup's head scan runs over SA positions), then their rows
+            std::vector<uint32_t> ord(Gl);
+            for (uint32_t g = 0; g < Gl; g++) ord[g] = g;
+            std::sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return h_lh[x] < h_lh[y]; });
+            h_lo.resize(2 * (size_t)Gl);
+            for (uint32_t k = 0; k < Gl; k++) {
+                h_lo[k] = h_lh[ord[k]];
+                h_lo[Gl + k] = ml;
+                ml += h_ls[ord[k]];
             }
+            HIPCHK(hipMemcpyAsync(w.lo.p, h_lo.data(), (size_t)2 * Gl * 4, hipMemcpyHostToDevice, s));
+            TIMED("large_elems", ml,
+                  launch_large_elems(P<uint32_t>(w.lo), P<uint32_t>(w.lo) + Gl, Gl, ml, P<uint32_t>(w.pos[0]),
+                                     P<uint32_t>(w.grp[0]), s));
+            HIPCHK(hipMemcpyAsync(w.orand, w.h_orand_init, 16, hipMemcpyHostToDevice, s));
+            TIMED("round_keys", ml,
+                  launch_round_keys(P<uint32_t>(w.pos[0]), P<uint32_t>(w.grp[0]), ml, SA, ISA, N, blk, d_bl, hh,
+                                    K[0], V[0], w.orand, s));
+            LAUNCHCHK();
+            HIPCHK(hipMemcpyAsync(w.h_orand, w.orand, 16, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            const uint64_t vary = w.h_orand[0] ^ w.h_orand[1];
+            for (uint32_t d = 0; d < 8; d++)
+                if ((vary >> (8 * d)) & 0xFFull) sh.s[sh.P++] = 8 * d;
         }
-        {
-            RegroupArgs g{};
-            g.K = K[src]; g.V = V[src]; g.pos = P<uint32_t>(w.pos[cur]); g.m = m; g.kshift = 0;
-            g.SA = SA; g.ISA = ISA; g.L = L; g.T = dT;
-            g.pos_out = P<uint32_t>(w.pos[cur ^ 1]); g.grp_out = P<uint32_t>(w.grp[cur ^ 1]); g.m_out = w.m_out;
-            g.st_sum = P<uint64_t>(w.st_sum); g.st_max = P<uint64_t>(w.st_max); g.counter = w.counters + 1;
-            g.epoch = w.epoch++;
-            TIMED("regroup", m, launch_regroup(g, s));
-        }
+        // sort and split; the next round's lists are built from zero
+        zero_counts();
+        if (Gs) TIMED("seg_sort", Es, launch_seg_round(1, sa, s));
         LAUNCHCHK();
-        HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
+        if (Gl) {
+            int src = 0;
+            if (sh.P) {
+                const uint32_t g2 = std::max(1u, std::min<uint32_t>(HIST_GRID, (ml + 8191) / 8192));
+                TIMED("hist_keys", ml, launch_hist_keys(K[0], ml, sh, P<uint32_t>(w.counts), g2, s));
+                TIMED("scan_digits", 256 * sh.P,
+                      launch_scan_digits(P<uint32_t>(w.counts), sh.P, MAXP, d_bl, 0, nullptr, P<uint32_t>(w.dbase), s));
+                LAUNCHCHK();
+                const uint32_t ntl = (ml + stile - 1) / stile;
+                for (uint32_t p = 0; p < sh.P; p++) {
+                    SortArgs a{};
+                    a.cfg = scfg;
+                    a.from_t = false;
+                    a.kin = K[src];
+                    a.vin = V[src];
+                    a.kout = K[src ^ 1];
+                    a.vout = V[src ^ 1];
+                    a.m = ml;
+                    a.shift = sh.s[p];
+                    a.dbase = P<uint32_t>(w.dbase) + p * 256;
+                    a.dstride = 0;
+                    a.tiling = flat;
+                    a.status = P<uint64_t>(w.status);
+                    a.counter = w.counters + 0;
+                    a.epoch = w.epoch++;
+                    TIMED("sort_pass", ml, launch_sort_pass(a, ntl, s));
+                    LAUNCHCHK();
+                    src ^= 1;
+                }
+            }
+            RegroupArgs g{};
+            g.K = K[src];
+            g.V = V[src];
+            g.pos = P<uint32_t>(w.pos[0]);
+            g.m = ml;
+            g.kshift = 0;
+            g.blk = nullptr;
+            g.SA = SA;
+            g.ISA = ISA;
+            g.L = L;
+            g.T = dT;
+            g.pos_out = P<uint32_t>(w.pos[1]);
+            g.grp_out = P<uint32_t>(w.grp[1]);
+            g.m_out = w.m_out;
+            g.st_sum = P<uint64_t>(w.st_sum);
+            g.st_max = P<uint64_t>(w.st_max);
+            g.counter = w.counters + 1;
+            g.epoch = w.epoch++;
+            TIMED("regroup", ml, launch_regroup(g, s));
+            TIMED("to_groups", ml,
+                  launch_elems_to_groups(P<uint32_t>(w.pos[1]), P<uint32_t>(w.grp[1]), w.m_out, ml,
+                                         P<uint32_t>(w.gh[cur ^ 1]), P<uint32_t>(w.gs[cur ^ 1]),
+                                         P<uint32_t>(w.lh[cur ^ 1]), P<uint32_t>(w.ls[cur ^ 1]), gcnt, s));
+            LAUNCHCHK();
+        }
         st.passes[round] = (int32_t)sh.P;
-        st.active[round] = m;
-        m = *w.h_m;
+        st.active[round] = (int64_t)Es + ml;
         cur ^= 1;
+        read_lists(cur, Gs, Gl);
         h *= 2;
     }
     st.rounds = round;
 
-    // 6. BWT bytes (the row bytes shifted around the primary row) and sampled indexes
-    TIMED("bwt", n, launch_bwt(dT, L, ISA, dU, n, s));
-    TIMED("samples", nsamp, launch_samples(ISA, r, nsamp, P<int32_t>(w.samples), s));
+    // 6. BWT bytes (the row bytes shifted around each block's primary row) and sampled indexes
+    TIMED("bwt", N, launch_bwt(dT, L, ISA, blk, d_bl, dU, N, s));
+    TIMED("samples", nsamp, launch_samples(ISA, d_bl, B, nsamp, P<int32_t>(w.samples), s));
     LAUNCHCHK();
     HIPCHK(hipEventRecord(w.ev1, s));
     HIPCHK(hipMemcpyAsync(I, w.samples.p, (size_t)nsamp * 4, hipMemcpyDeviceToHost, s));
@@ -369,38 +555,75 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t 
 
 // ------------------------------------------------------------------ inverse
 
-void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, uint32_t n, uint32_t Pidx, hipStream_t s) {
+// bl: one BlockInfo per block with off, n (>= 2) and P (primary index) filled in; packed BWTs dL, output dOut (may
+// be dL).
+void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockInfo> &bl, uint32_t N, hipStream_t s) {
+    const uint32_t B = (uint32_t)bl.size();
     kbwt_stats &st = g_stats;
+    st.n = (int32_t)N;
+    st.blocks = (int32_t)B;
     // one splitter row in D = 2^dshift; the first cap bytes of each sublist are staged during the counting walk
     const uint32_t dshift = env_u32("KBWT_INV_DSHIFT", 6, 2, 12);
     const uint32_t cap = env_u32("KBWT_INV_CAP", 4u << dshift, 1, 1u << 16);
-    const size_t N = std::max<size_t>(n, 64);
     const uint32_t dmask = (1u << dshift) - 1u;
-    const uint32_t nreg = (n >> dshift) + 1;
-    const uint32_t S = nreg + ((Pidx & dmask) != 0 ? 1 : 0);
-    grow(w.V0, N * 4);                                 // lf
-    grow(w.pos[0], ((size_t)S + 1) * 4);               // sublist lengths
-    grow(w.pos[1], ((size_t)S + 1) * 4);               // continuation rows
-    grow(w.K0, ((size_t)S + 1) * 8);                   // links (ping)
-    grow(w.K1, ((size_t)S + 1) * 8);                   // links (pong)
+    uint32_t S = 0;
+    for (uint32_t b = 0; b < B; b++) {
+        bl[b].roff = bl[b].off + b;
+        bl[b].spoff = S;
+        bl[b].nreg = (bl[b].n >> dshift) + 1;
+        S += bl[b].nreg + ((bl[b].P & dmask) != 0 ? 1u : 0u);
+    }
+    st.splitters = (int32_t)S;
+    const size_t NN = std::max<size_t>(N, 64);
+    grow(w.V0, NN * 4);                        // lf
+    grow(w.pos[0], ((size_t)S + 1) * 4);       // sublist lengths
+    grow(w.pos[1], ((size_t)S + 1) * 4);       // continuation rows
+    grow(w.K0, ((size_t)S + 1) * 8);           // links (ping)
+    grow(w.K1, ((size_t)S + 1) * 8);           // links (pong)
     grow(w.stage, (size_t)S * cap);
-    const uint32_t ntiles = (uint32_t)((N + LF_TILE - 1) / LF_TILE);
-    grow(w.status, (size_t)ntiles * 256 * 8, true);
+    const size_t ntiles_max = (NN + LF_TILE - 1) / LF_TILE + B;
+    grow(w.status, ntiles_max * 256 * 8, true);
+    grow(w.counts, (size_t)B * MAXP * 256 * 4, true);
+    grow(w.cbase, (size_t)B * 256 * 4);
+    grow(w.bad, (size_t)B * 4);
+    if (B > 1) grow(w.blk, NN * 2);
+
+    // metadata layout: blocks | hist tiles | lf tiles | walk tiles; the bad flags come back behind it
+    const size_t o_ht = align16(B * sizeof(BlockInfo)), o_lt = o_ht + (B + 1) * 4, o_wt = o_lt + (B + 1) * 4;
+    const size_t bytes = o_wt + (B + 1) * 4;
+    uint8_t *hm = (uint8_t *)host_staging(w, bytes + (size_t)B * 4);
+    grow(w.meta, bytes);
+    uint8_t *dm = P<uint8_t>(w.meta);
+    uint32_t *h_bad = (uint32_t *)(hm + align16(bytes));
+    memcpy(hm, bl.data(), B * sizeof(BlockInfo));
+    const uint32_t nht = fill_tiles((uint32_t *)(hm + o_ht), B, [&](uint32_t b) { return (bl[b].n + HIST_TILE - 1) / HIST_TILE; });
+    const uint32_t nlt = fill_tiles((uint32_t *)(hm + o_lt), B, [&](uint32_t b) { return (bl[b].n + LF_TILE - 1) / LF_TILE; });
+    const uint32_t nwt = fill_tiles((uint32_t *)(hm + o_wt), B, [&](uint32_t b) {
+        return (bl[b].nreg + ((bl[b].P & dmask) != 0 ? 1u : 0u) + WALK_TILE - 1) / WALK_TILE;
+    });
+    HIPCHK(hipMemcpyAsync(dm, hm, bytes, hipMemcpyHostToDevice, s));
+    const BlockInfo *d_bl = (const BlockInfo *)dm;
+    const Tiling thist{(const uint32_t *)(dm + o_ht), d_bl, B};
+    const Tiling tlf{(const uint32_t *)(dm + o_lt), d_bl, B};
+    const Tiling twalk{(const uint32_t *)(dm + o_wt), d_bl, B};
+    const uint16_t *blk = B > 1 ? P<uint16_t>(w.blk) : nullptr;
 
     HIPCHK(hipEventRecord(w.ev0, s));
-    reset_counts(w, s);
-    const uint32_t g1 = hist_grid(n);
-    TIMED("byte_hist", n, launch_byte_hist(dL, n, w.counts, g1, s));
-    TIMED("scan_digits", 256, launch_scan_digits(w.counts, 1, nullptr, w.cbase, 1, s));
+    launch_fill_u8(P<uint8_t>(w.counts), 0, (size_t)B * MAXP * 256 * 4, s);
+    if (B > 1) TIMED("fill_blk", N, launch_fill_blk(P<uint16_t>(w.blk), d_bl, B, N, s));
+    TIMED("byte_hist", N, launch_byte_hist(dL, thist, nht, P<uint32_t>(w.counts), s));
+    TIMED("scan_digits", 256 * B,
+          launch_scan_digits(P<uint32_t>(w.counts), B, 1, d_bl, 2, nullptr, P<uint32_t>(w.cbase), s));
     uint32_t *lf = P<uint32_t>(w.V0);
-    TIMED("lf_pass", n, launch_lf_pass(dL, n, w.cbase, lf, P<uint64_t>(w.status), w.counters + 2, w.epoch++, s));
+    TIMED("lf_pass", N,
+          launch_lf_pass(dL, tlf, nlt, P<uint32_t>(w.cbase), lf, P<uint64_t>(w.status), w.counters + 2, w.epoch++, s));
     LAUNCHCHK();
 
-    st.splitters = (int32_t)S;
     uint64_t *link[2] = {P<uint64_t>(w.K0), P<uint64_t>(w.K1)};
     uint32_t *len = P<uint32_t>(w.pos[0]), *cont = P<uint32_t>(w.pos[1]);
     uint8_t *stage = P<uint8_t>(w.stage);
-    TIMED("walk_count", n, launch_walk_count(lf, n, Pidx, dshift, nreg, S, w.cbase, link[0], len, stage, cap, cont, s));
+    TIMED("walk_count", N,
+          launch_walk_count(lf, twalk, nwt, dshift, S, P<uint32_t>(w.cbase), link[0], len, stage, cap, cont, s));
     LAUNCHCHK();
     const uint32_t iters = ceil_log2(S + 1) + 1;
     int cur = 0;
@@ -408,21 +631,164 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, uint32_t n, uint32_
         TIMED("rank_jump", S, launch_rank_jump(link[cur], link[cur ^ 1], S, s));
         cur ^= 1;
     }
+    launch_check_lists(link[cur], d_bl, B, S, P<uint32_t>(w.bad), s);
+    TIMED("walk_emit", N,
+          launch_walk_emit(lf, twalk, nwt, dshift, S, link[cur], len, P<uint32_t>(w.cbase), stage, cap, cont,
+                           P<uint32_t>(w.bad), dOut, s));
     LAUNCHCHK();
-    HIPCHK(hipMemcpyAsync(w.h_link, link[cur], 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(h_bad, w.bad.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
-    const uint64_t l0 = *w.h_link;
-    const bool valid = (uint32_t)l0 == S && (uint32_t)(l0 >> 32) == n;
-    if (valid)
-        TIMED("walk_emit", n, launch_walk_emit(lf, n, Pidx, S, link[cur], len, w.cbase, stage, cap, cont, dOut, s));
-    else launch_fill_u8(dOut, 0, n, s);   // not a BWT: deterministic bytes that fail the caller's checksum
-    LAUNCHCHK();
+    bool anybad = false;
+    for (uint32_t b = 0; b < B; b++) anybad |= h_bad[b] != 0;
+    if (anybad) {   // strings that are no BWT decode to zeros: deterministic bytes that fail the caller's checksum
+        launch_zero_bad(dOut, blk, P<uint32_t>(w.bad), N, s);
+        LAUNCHCHK();
+    }
     HIPCHK(hipEventRecord(w.ev1, s));
     HIPCHK(hipStreamSynchronize(s));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, w.ev0, w.ev1));
     st.ms_inv = ms;
     prof_collect(w);
+}
+
+// ------------------------------------------------------------------ batch plumbing
+
+bool forward_args_ok(int32_t n, int32_t r) { return n >= 0 && r >= 2 && (r & (r - 1)) == 0; }
+bool inverse_args_ok(int32_t n, int32_t r) { return n >= 0 && (r == n || (r >= 2 && (r & (r - 1)) == 0)); }
+inline uint32_t nsamples(int32_t n, int32_t r) { return n > 0 ? (uint32_t)((n - 1) / r + 1) : 1u; }
+
+// Validates a batch: increasing, non-overlapping blocks, at most 65535, total length < 2^31.
+bool batch_ok(const kbwt_seg *segs, int32_t count, bool fwd) {
+    if (!segs || count < 0 || count > 65535) return false;
+    int64_t end = 0, total = 0;
+    for (int32_t b = 0; b < count; b++) {
+        const kbwt_seg &g = segs[b];
+        if (g.offset < end) return false;
+        if (fwd ? !forward_args_ok(g.n, g.r) : !inverse_args_ok(g.n, g.r)) return false;
+        end = g.offset + g.n;
+        total += g.n;
+    }
+    return total < ((int64_t)1 << 31) - 64;
+}
+
+// True when every block has n >= 2, they sit back to back, and the first starts 16-byte aligned.
+bool packed(const uint8_t *base, const kbwt_seg *segs, int32_t count) {
+    int64_t at = -1;
+    for (int32_t b = 0; b < count; b++) {
+        if (segs[b].n <= 1) return false;
+        if (at >= 0 && segs[b].offset != at) return false;
+        at = segs[b].offset + segs[b].n;
+    }
+    return count == 0 || (((uintptr_t)(base + segs[0].offset)) & 15) == 0;
+}
+
+// Forward on a batch whose bytes sit in device memory (dT, dU device pointers; I host).
+void forward_batch_dev(Workspace &w, const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count, int32_t *I,
+                       hipStream_t s) {
+    std::vector<uint32_t> soff(count + 1, 0);
+    for (int32_t b = 0; b < count; b++) soff[b + 1] = soff[b] + nsamples(segs[b].n, segs[b].r);
+    std::vector<BlockInfo> bl;
+    std::vector<int32_t> which;
+    uint32_t N = 0, nsamp = 0;
+    for (int32_t b = 0; b < count; b++) {
+        if (segs[b].n <= 1) continue;
+        BlockInfo bi{};
+        bi.off = N;
+        bi.n = (uint32_t)segs[b].n;
+        bi.r = (uint32_t)segs[b].r;
+        bi.soff = nsamp;
+        bl.push_back(bi);
+        which.push_back(b);
+        N += bi.n;
+        nsamp += nsamples(segs[b].n, segs[b].r);
+    }
+    // the BWT reads T at random while writing U: in place needs a copy, and so does a batch that is not packed
+    const bool direct = packed(dT, segs, count) && dT != dU;
+    std::vector<int32_t> Id(std::max<uint32_t>(nsamp, 1));
+    if (!bl.empty()) {
+        const uint8_t *src = dT + segs[0].offset;
+        uint8_t *dst = dU + segs[0].offset;
+        if (!direct) {
+            grow(w.T, (size_t)N + 64);
+            grow(w.U, (size_t)N + 64);
+            for (size_t k = 0; k < bl.size(); k++)
+                HIPCHK(hipMemcpyAsync(P<uint8_t>(w.T) + bl[k].off, dT + segs[which[k]].offset, bl[k].n,
+                                      hipMemcpyDeviceToDevice, s));
+            src = P<uint8_t>(w.T);
+            dst = P<uint8_t>(w.U);
+        }
+        forward(w, src, dst, bl, N, Id.data(), nsamp, s);
+        if (!direct)
+            for (size_t k = 0; k < bl.size(); k++)
+                HIPCHK(hipMemcpyAsync(dU + segs[which[k]].offset, P<uint8_t>(w.U) + bl[k].off, bl[k].n,
+                                      hipMemcpyDeviceToDevice, s));
+    }
+    for (int32_t b = 0; b < count; b++)
+        if (segs[b].n <= 1) {
+            if (segs[b].n == 1)
+                HIPCHK(hipMemcpyAsync(dU + segs[b].offset, dT + segs[b].offset, 1, hipMemcpyDeviceToDevice, s));
+            I[soff[b]] = segs[b].n;
+        }
+    for (size_t k = 0; k < bl.size(); k++)
+        memcpy(I + soff[which[k]], Id.data() + bl[k].soff, nsamples((int32_t)bl[k].n, (int32_t)bl[k].r) * 4);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+// Inverse on a batch in device memory; dT == dU allowed.
+void inverse_batch_dev(Workspace &w, const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count,
+                       const int32_t *I, hipStream_t s) {
+    std::vector<uint32_t> soff(count + 1, 0);
+    for (int32_t b = 0; b < count; b++) soff[b + 1] = soff[b] + nsamples(segs[b].n, segs[b].r);
+    std::vector<BlockInfo> bl;
+    std::vector<int32_t> which;
+    uint32_t N = 0;
+    for (int32_t b = 0; b < count; b++) {
+        if (segs[b].n <= 1) continue;
+        BlockInfo bi{};
+        bi.off = N;
+        bi.n = (uint32_t)segs[b].n;
+        bi.P = (uint32_t)I[soff[b]];
+        bl.push_back(bi);
+        which.push_back(b);
+        N += bi.n;
+    }
+    // the walks read only the LF table, so the output may overwrite the input
+    const bool direct = packed(dT, segs, count) && (dT == dU || packed(dU, segs, count));
+    if (!bl.empty()) {
+        const uint8_t *src = dT + segs[0].offset;
+        uint8_t *dst = dU + segs[0].offset;
+        if (!direct) {
+            grow(w.T, (size_t)N + 64);
+            for (size_t k = 0; k < bl.size(); k++)
+                HIPCHK(hipMemcpyAsync(P<uint8_t>(w.T) + bl[k].off, dT + segs[which[k]].offset, bl[k].n,
+                                      hipMemcpyDeviceToDevice, s));
+            src = P<uint8_t>(w.T);
+            dst = P<uint8_t>(w.T);
+        }
+        inverse(w, src, dst, bl, N, s);
+        if (!direct)
+            for (size_t k = 0; k < bl.size(); k++)
+                HIPCHK(hipMemcpyAsync(dU + segs[which[k]].offset, P<uint8_t>(w.T) + bl[k].off, bl[k].n,
+                                      hipMemcpyDeviceToDevice, s));
+    }
+    for (int32_t b = 0; b < count; b++)
+        if (segs[b].n == 1)
+            HIPCHK(hipMemcpyAsync(dU + segs[b].offset, dT + segs[b].offset, 1, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+bool samples_ok(const kbwt_seg *segs, int32_t count, const int32_t *I) {
+    for (int32_t b = 0, k = 0; b < count; k += (int32_t)nsamples(segs[b].n, segs[b].r), b++) {
+        const int32_t n = segs[b].n;
+        if (n <= 1) {
+            if (I[k] != n) return false;
+            continue;
+        }
+        for (int64_t t = 0; t <= (n - 1) / segs[b].r; ++t)
+            if (I[k + t] <= 0 || I[k + t] > n) return false;
+    }
+    return true;
 }
 
 }  // namespace
@@ -460,7 +826,12 @@ int32_t kolmo_sais_bwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, 
         grow(w.T, (size_t)n + 64);
         grow(w.U, (size_t)n + 64);
         HIPCHK(hipMemcpyAsync(w.T.p, T, (size_t)n, hipMemcpyHostToDevice, w.stream));
-        forward(w, P<uint8_t>(w.T), P<uint8_t>(w.U), (uint32_t)n, (uint32_t)r, I, w.stream);
+        std::vector<BlockInfo> bl(1);
+        bl[0].off = 0;
+        bl[0].n = (uint32_t)n;
+        bl[0].r = (uint32_t)r;
+        bl[0].soff = 0;
+        forward(w, P<uint8_t>(w.T), P<uint8_t>(w.U), bl, (uint32_t)n, I, nsamples(n, r), w.stream);
         HIPCHK(hipMemcpyAsync(U, w.U.p, (size_t)n, hipMemcpyDeviceToHost, w.stream));
         HIPCHK(hipStreamSynchronize(w.stream));
     });
@@ -483,7 +854,11 @@ int32_t kolmo_sais_unbwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n
         std::lock_guard<std::mutex> lk(w.mu);
         grow(w.T, (size_t)n + 64);
         HIPCHK(hipMemcpyAsync(w.T.p, T, (size_t)n, hipMemcpyHostToDevice, w.stream));
-        inverse(w, P<uint8_t>(w.T), P<uint8_t>(w.T), (uint32_t)n, (uint32_t)I[0], w.stream);
+        std::vector<BlockInfo> bl(1);
+        bl[0].off = 0;
+        bl[0].n = (uint32_t)n;
+        bl[0].P = (uint32_t)I[0];
+        inverse(w, P<uint8_t>(w.T), P<uint8_t>(w.T), bl, (uint32_t)n, w.stream);
         HIPCHK(hipMemcpyAsync(U, w.T.p, (size_t)n, hipMemcpyDeviceToHost, w.stream));
         HIPCHK(hipStreamSynchronize(w.stream));
     });
@@ -494,49 +869,101 @@ int32_t kolmo_sais_unbwt(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, in
 }
 
 int32_t kbwt_dev_bwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, int32_t *I, void *stream) {
-    if (dT == NULL || dU == NULL || n < 0 || r < 2 || (r & (r - 1)) != 0 || I == NULL) return -1;
+    if (dT == NULL || dU == NULL || !forward_args_ok(n, r) || I == NULL) return -1;
     return guarded([&] {
         Workspace &w = workspace();
         std::lock_guard<std::mutex> lk(w.mu);
         hipStream_t s = stream ? (hipStream_t)stream : w.stream;
-        if (n <= 1) {
-            if (n == 1) HIPCHK(hipMemcpyAsync(dU, dT, 1, hipMemcpyDeviceToDevice, s));
-            I[0] = n;
-            HIPCHK(hipStreamSynchronize(s));
-            return;
-        }
-        const uint8_t *src = dT;
-        if (dT == dU || ((uintptr_t)dT & 15)) {   // in place, or not 16-byte aligned: work from a copy
-            grow(w.T, (size_t)n + 64);
-            HIPCHK(hipMemcpyAsync(w.T.p, dT, (size_t)n, hipMemcpyDeviceToDevice, s));
-            src = P<uint8_t>(w.T);
-        }
-        forward(w, src, dU, (uint32_t)n, (uint32_t)r, I, s);
+        kbwt_seg g{0, n, r};
+        forward_batch_dev(w, dT, dU, &g, 1, I, s);
     });
 }
 
 int32_t kbwt_dev_unbwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, const int32_t *I, void *stream) {
-    if (dT == NULL || dU == NULL || n < 0 || ((r != n) && ((r < 2) || ((r & (r - 1)) != 0))) || I == NULL) return -1;
-    if (n <= 1 && I[0] != n) return -1;
-    if (n > 1)
-        for (int64_t t = 0; t <= (n - 1) / r; ++t)
-            if (I[t] <= 0 || I[t] > n) return -1;
+    if (dT == NULL || dU == NULL || !inverse_args_ok(n, r) || I == NULL) return -1;
+    kbwt_seg g{0, n, r};
+    if (!samples_ok(&g, 1, I)) return -1;
     return guarded([&] {
         Workspace &w = workspace();
         std::lock_guard<std::mutex> lk(w.mu);
         hipStream_t s = stream ? (hipStream_t)stream : w.stream;
-        if (n <= 1) {
-            if (n == 1) HIPCHK(hipMemcpyAsync(dU, dT, 1, hipMemcpyDeviceToDevice, s));
-            HIPCHK(hipStreamSynchronize(s));
-            return;
+        inverse_batch_dev(w, dT, dU, &g, 1, I, s);
+    });
+}
+
+int32_t kbwt_dev_bwt_aux_batch(const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count, int32_t *I,
+                               void *stream) {
+    if (dT == NULL || dU == NULL || I == NULL || !batch_ok(segs, count, true)) return -1;
+    return guarded([&] {
+        Workspace &w = workspace();
+        std::lock_guard<std::mutex> lk(w.mu);
+        hipStream_t s = stream ? (hipStream_t)stream : w.stream;
+        forward_batch_dev(w, dT, dU, segs, count, I, s);
+    });
+}
+
+int32_t kbwt_dev_unbwt_aux_batch(const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count,
+                                 const int32_t *I, void *stream) {
+    if (dT == NULL || dU == NULL || I == NULL || !batch_ok(segs, count, false)) return -1;
+    if (!samples_ok(segs, count, I)) return -1;
+    return guarded([&] {
+        Workspace &w = workspace();
+        std::lock_guard<std::mutex> lk(w.mu);
+        hipStream_t s = stream ? (hipStream_t)stream : w.stream;
+        inverse_batch_dev(w, dT, dU, segs, count, I, s);
+    });
+}
+
+// Host-pointer batches: the span [first offset, last end) goes to the device once, keeping the offsets.
+int32_t kbwt_bwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs, int32_t count, int32_t *I) {
+    if (T == NULL || U == NULL || I == NULL || !batch_ok(segs, count, true)) return -1;
+    if (count == 0) return 0;
+    return guarded([&] {
+        Workspace &w = workspace();
+        std::lock_guard<std::mutex> lk(w.mu);
+        const int64_t lo = segs[0].offset, hi = segs[count - 1].offset + segs[count - 1].n;
+        Buf in, out;
+        HIPCHK(hipMalloc(&in.p, (size_t)(hi - lo) + 64));
+        HIPCHK(hipMalloc(&out.p, (size_t)(hi - lo) + 64));
+        try {
+            HIPCHK(hipMemcpyAsync(in.p, T + lo, (size_t)(hi - lo), hipMemcpyHostToDevice, w.stream));
+            std::vector<kbwt_seg> g(segs, segs + count);
+            for (auto &x : g) x.offset -= lo;
+            forward_batch_dev(w, P<uint8_t>(in), P<uint8_t>(out), g.data(), count, I, w.stream);
+            HIPCHK(hipMemcpyAsync(U + lo, out.p, (size_t)(hi - lo), hipMemcpyDeviceToHost, w.stream));
+            HIPCHK(hipStreamSynchronize(w.stream));
+        } catch (...) {
+            (void)hipFree(in.p);
+            (void)hipFree(out.p);
+            throw;
         }
-        const uint8_t *src = dT;
-        if ((uintptr_t)dT & 15) {   // the byte histogram reads whole words: work from an aligned copy
-            grow(w.T, (size_t)n + 64);
-            HIPCHK(hipMemcpyAsync(w.T.p, dT, (size_t)n, hipMemcpyDeviceToDevice, s));
-            src = P<uint8_t>(w.T);
+        HIPCHK(hipFree(in.p));
+        HIPCHK(hipFree(out.p));
+    });
+}
+
+int32_t kbwt_unbwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs, int32_t count, const int32_t *I) {
+    if (T == NULL || U == NULL || I == NULL || !batch_ok(segs, count, false)) return -1;
+    if (!samples_ok(segs, count, I)) return -1;
+    if (count == 0) return 0;
+    return guarded([&] {
+        Workspace &w = workspace();
+        std::lock_guard<std::mutex> lk(w.mu);
+        const int64_t lo = segs[0].offset, hi = segs[count - 1].offset + segs[count - 1].n;
+        Buf io;
+        HIPCHK(hipMalloc(&io.p, (size_t)(hi - lo) + 64));
+        try {
+            HIPCHK(hipMemcpyAsync(io.p, T + lo, (size_t)(hi - lo), hipMemcpyHostToDevice, w.stream));
+            std::vector<kbwt_seg> g(segs, segs + count);
+            for (auto &x : g) x.offset -= lo;
+            inverse_batch_dev(w, P<uint8_t>(io), P<uint8_t>(io), g.data(), count, I, w.stream);
+            HIPCHK(hipMemcpyAsync(U + lo, io.p, (size_t)(hi - lo), hipMemcpyDeviceToHost, w.stream));
+            HIPCHK(hipStreamSynchronize(w.stream));
+        } catch (...) {
+            (void)hipFree(io.p);
+            throw;
         }
-        inverse(w, src, dU, (uint32_t)n, (uint32_t)I[0], s);
+        HIPCHK(hipFree(io.p));
     });
 }
 
@@ -595,8 +1022,11 @@ void kbwt_release(void) {
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
     Workspace &w = g_ws[dev];
     std::lock_guard<std::mutex> lk(w.mu);
-    Buf *bufs[] = {&w.T, &w.U, &w.K0, &w.K1, &w.V0, &w.V1, &w.SA, &w.ISA, &w.L, &w.pos[0], &w.pos[1],
-                   &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.st_max, &w.samples, &w.stage};
+    Buf *bufs[] = {&w.T,       &w.U,      &w.K0,    &w.K1,     &w.V0,     &w.V1,     &w.SA,     &w.ISA,    &w.L,
+                   &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.st_max, &w.samples,
+                   &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
+                   &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
+                   &w.go,      &w.sec,    &w.lo};
     for (Buf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;

@@ -1,4 +1,7 @@
 // Internal interface between the kernels (kbwt_kernels.hip) and the host orchestration (kbwt_host.hip).
+//
+// Every call works on a batch of B blocks packed back to back: block b owns bytes, suffixes and suffix-array rows
+// [off, off + n) of the packed index space. A single-block call is the batch B = 1 (blk == nullptr).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -7,35 +10,68 @@
 namespace kbwt {
 
 constexpr int MAXP = 8;                      // 8-bit digits of a 64-bit key
-constexpr int SORT_BLOCK = 256, SORT_ITEMS = 16, SORT_TILE = SORT_BLOCK * SORT_ITEMS;
 constexpr int RG_BLOCK = 256, RG_ITEMS = 8, RG_TILE = RG_BLOCK * RG_ITEMS;
 constexpr int LF_BLOCK = 256, LF_ITEMS = 32, LF_TILE = LF_BLOCK * LF_ITEMS;
+constexpr int HIST_TILE = 4096;              // bytes per tile of the histogram kernels
 constexpr int HIST_GRID = 512;               // workgroups of the histogram kernels
+constexpr int WALK_TILE = 256;               // splitters per workgroup of the inverse walks
+constexpr uint32_t MIN_TILE = 2048;          // smallest tile of any tiled kernel (sizes the lookback arrays)
+constexpr int SEG_TILE = 1024;               // doubling rounds: groups of up to SEG_TILE rows sort inside one workgroup
+
+// One block of a batch as the kernels see it.
+struct BlockInfo {
+    uint32_t off;      // first packed index of the block
+    uint32_t n;        // length
+    uint32_t w, h0;    // forward: bits of the symbol code, symbols in the first key
+    uint32_t r;        // forward: sampling interval
+    uint32_t soff;     // forward: first entry of the block's sampled indexes
+    uint32_t P;        // inverse: primary index (the end marker's row, 1..n)
+    uint32_t roff;     // inverse: first global row (off + block number; a block has n + 1 rows)
+    uint32_t spoff;    // inverse: first splitter
+    uint32_t nreg;     // inverse: splitters at rows 0, D, 2D, ... of the block
+};
+
+// Tiles of a tiled kernel over the blocks of a batch: block b owns tiles [first[b], first[b+1]). A kernel over one
+// flat range of m elements uses first == nullptr.
+struct Tiling {
+    const uint32_t *first;
+    const BlockInfo *blocks;
+    uint32_t B;
+};
 
 struct Shifts {
     uint32_t P;
     uint32_t s[MAXP];
 };
 
+struct SortCfg {
+    int block, items, window;
+};
+constexpr int SORT_NCFG = 7;
+extern const SortCfg SORT_CFGS[SORT_NCFG];
+
 struct SortArgs {
+    int cfg;
     const uint64_t *kin;
     const uint32_t *vin;
     uint64_t *kout;
     uint32_t *vout;
     uint32_t m, shift;
-    const uint32_t *dbase;
+    const uint32_t *dbase;        // digit bases: dbase[b * dstride + d] (per block), or dbase[d]
+    uint32_t dstride;
+    Tiling tiling;                // per-block segmented sort when tiling.first != nullptr
     uint64_t *status;
     uint32_t *counter;
     uint32_t epoch;
-    bool from_t;
-    const uint8_t *T, *code;
-    uint32_t w, h0;
+    bool from_t;                  // first pass: keys from T with each block's code table
+    const uint8_t *T, *codes;
 };
 
 struct RegroupArgs {
     const uint64_t *K;
     const uint32_t *V, *pos;
     uint32_t m, kshift;
+    const uint16_t *blk;          // block of every packed index, or nullptr for one block
     uint32_t *SA, *ISA;
     uint8_t *L;
     const uint8_t *T;
@@ -44,28 +80,56 @@ struct RegroupArgs {
     uint32_t *counter, epoch;
 };
 
-void launch_byte_hist(const uint8_t *T, uint32_t n, uint32_t *counts, uint32_t grid, hipStream_t s);
-void launch_scan_digits(uint32_t *counts, uint32_t nsets, uint32_t *tot, uint32_t *excl, uint32_t base,
+struct SegRoundArgs {
+    const uint32_t *gh, *go;      // the small groups: heads and first-row offsets (prefix sums of sizes)
+    uint32_t G, E;                // groups, rows
+    uint32_t *SA, *ISA, *sec;
+    uint8_t *L;
+    const uint8_t *T;
+    const uint16_t *blk;
+    const BlockInfo *blocks;
+    uint32_t n, h;
+    uint32_t *nh, *ns, *ncnt;     // next round's small list and its counter
+};
+
+void launch_elems_to_groups(const uint32_t *pos, const uint32_t *grp, const uint32_t *mp, uint32_t mmax,
+                            uint32_t *sh, uint32_t *ss, uint32_t *lh, uint32_t *ls, uint32_t *cnt, hipStream_t s);
+void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *tot, uint64_t *status,
+                        uint32_t *counter, uint32_t epoch, hipStream_t s);
+void launch_large_elems(const uint32_t *lh, const uint32_t *lo, uint32_t G, uint32_t m, uint32_t *pos, uint32_t *grp,
                         hipStream_t s);
-void launch_hist_T(const uint8_t *T, uint32_t n, const uint8_t *code, uint32_t w, uint32_t h0, uint32_t P,
-                   uint32_t *counts, uint32_t grid, hipStream_t s);
+void launch_seg_round(int mode, const SegRoundArgs &a, hipStream_t s);
+
+void launch_byte_hist(const uint8_t *T, Tiling tl, uint32_t ntiles, uint32_t *counts, hipStream_t s);
+void launch_scan_digits(uint32_t *counts, uint32_t nsets, uint32_t sets_per_block, const BlockInfo *blocks,
+                        int base_mode, uint32_t *tot, uint32_t *excl, hipStream_t s);
+void launch_hist_T(const uint8_t *T, Tiling tl, uint32_t ntiles, const uint8_t *codes, uint32_t P, uint32_t *counts,
+                   hipStream_t s);
 void launch_hist_keys(const uint64_t *K, uint32_t m, const Shifts &sh, uint32_t *counts, uint32_t grid,
                       hipStream_t s);
-void launch_sort_pass(const SortArgs &a, hipStream_t s);
+uint32_t sort_tile(int cfg);
+void launch_sort_pass(const SortArgs &a, uint32_t ntiles, hipStream_t s);
 void launch_regroup(const RegroupArgs &a, hipStream_t s);
 void launch_round_keys(const uint32_t *pos, const uint32_t *grp, uint32_t m, const uint32_t *SA, const uint32_t *ISA,
-                       uint32_t n, uint32_t h, uint64_t *K, uint32_t *V, unsigned long long *orand, hipStream_t s);
-void launch_bwt(const uint8_t *T, const uint8_t *L, const uint32_t *ISA, uint8_t *U, uint32_t n, hipStream_t s);
-void launch_samples(const uint32_t *ISA, uint32_t r, uint32_t count, int32_t *I, hipStream_t s);
-void launch_lf_pass(const uint8_t *L, uint32_t n, const uint32_t *cbase, uint32_t *lf, uint64_t *status,
-                    uint32_t *counter, uint32_t epoch, hipStream_t s);
-void launch_walk_count(const uint32_t *lf, uint32_t n, uint32_t P, uint32_t dshift, uint32_t nreg, uint32_t S,
+                       uint32_t n, const uint16_t *blk, const BlockInfo *blocks, uint32_t h, uint64_t *K, uint32_t *V,
+                       unsigned long long *orand, hipStream_t s);
+void launch_bwt(const uint8_t *T, const uint8_t *L, const uint32_t *ISA, const uint16_t *blk, const BlockInfo *blocks,
+                uint8_t *U, uint32_t N, hipStream_t s);
+void launch_samples(const uint32_t *ISA, const BlockInfo *blocks, uint32_t B, uint32_t count, int32_t *I,
+                    hipStream_t s);
+void launch_fill_blk(uint16_t *blk, const BlockInfo *blocks, uint32_t B, uint32_t N, hipStream_t s);
+void launch_lf_pass(const uint8_t *L, Tiling tl, uint32_t ntiles, const uint32_t *cbase, uint32_t *lf,
+                    uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s);
+void launch_walk_count(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
                        const uint32_t *cbase, uint64_t *link, uint32_t *len, uint8_t *stage, uint32_t cap,
                        uint32_t *cont, hipStream_t s);
 void launch_rank_jump(const uint64_t *in, uint64_t *out, uint32_t S, hipStream_t s);
-void launch_walk_emit(const uint32_t *lf, uint32_t n, uint32_t P, uint32_t S, const uint64_t *link,
-                      const uint32_t *len, const uint32_t *cbase, const uint8_t *stage, uint32_t cap,
-                      const uint32_t *cont, uint8_t *out, hipStream_t s);
+void launch_check_lists(const uint64_t *link, const BlockInfo *blocks, uint32_t B, uint32_t S, uint32_t *bad,
+                        hipStream_t s);
+void launch_walk_emit(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
+                      const uint64_t *link, const uint32_t *len, const uint32_t *cbase, const uint8_t *stage,
+                      uint32_t cap, const uint32_t *cont, const uint32_t *bad, uint8_t *out, hipStream_t s);
+void launch_zero_bad(uint8_t *out, const uint16_t *blk, const uint32_t *bad, uint32_t N, hipStream_t s);
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 void launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s);
 

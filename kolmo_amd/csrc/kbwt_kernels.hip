@@ -1,112 +1,183 @@
 // Kernels of the forward suffix sort (prefix doubling), the BWT emission and the inverse BWT, for gfx950.
 //
-// Forward, for a block T[0..n):
-//   1. k_byte_hist + k_reduce_hist: byte histogram; the host turns it into a dense order-preserving symbol code of
-//      w = ceil(log2 sigma) bits and picks h0 symbols per first key (h0*w <= 64 bits).
-//   2. k_hist_T + k_scan_digits: the 8-bit digit histograms of every first key, in one read of T; exclusive scans
-//      give each digit's global bucket offsets (onesweep).
-//   3. k_sort_pass: one stable LSD pass per varying digit. The first pass builds keys from T directly. A tile of
-//      BLOCK*ITEMS elements is ranked by wave64 ballot multisplit, its digit offsets come from a decoupled lookback
-//      over earlier tiles, and it is written through LDS so equal digits leave as contiguous runs.
-//   4. k_regroup: group heads (rank = SA position of the group's first row) by a max-scan, ISA[suffix] = head,
-//      and compaction of the rows still in groups of two or more, all with decoupled lookback.
-//   5. per doubling round with offset h: k_round_keys gathers (group head, ISA[i+h] + h or n-1-i past the end) for
-//      the active rows, the same radix passes sort them, k_regroup splits the groups.
-//   6. k_bwt + k_samples: U in libsais' convention and I[k] = 1 + ISA[k*r].
-// Inverse, for a BWT L[0..n) with primary index P (the end marker's row in the full n+1-row matrix):
-//   1. byte histogram -> C[c] = 1 + #bytes < c.
+// Every kernel works on a batch of blocks packed back to back (one block is the batch of one). Tiled kernels map a
+// tile id to its block through the block's first tile (Tiling); lookback chains never cross a block.
+//
+// Forward, per batch:
+//   1. k_byte_hist + k_scan_digits: byte histogram of every block; the host turns each into a dense order-preserving
+//      symbol code of w = ceil(log2 sigma) bits and picks h0 symbols per first key (h0*w <= 56 bits).
+//   2. k_hist_T + k_scan_digits: the 8-bit digit histograms of every first key of every block, in one read of T;
+//      exclusive scans give each block's digit buckets (onesweep).
+//   3. k_sort_pass: one stable LSD pass per digit, segmented by block. The first pass builds the keys from T: the
+//      symbol codes above the BWT byte T[i-1]. A tile of BLOCK*ITEMS elements is ranked by wave64 ballot multisplit,
+//      its digit offsets come from a decoupled lookback over earlier tiles of its block, and it leaves through LDS
+//      so that equal digits go out as contiguous runs.
+//   4. k_regroup: group heads (rank = SA position of the group's first row) by a max-scan, ISA[suffix] = head, the
+//      row's BWT byte, and compaction of the rows still in groups of two or more, with decoupled lookback.
+//   5. per doubling round with offset h: k_round_keys gathers (group head, ISA[i+h] + h, or the distance to the
+//      block's end when i+h lies past it) for the active rows, the same radix passes sort them (one flat range: the
+//      heads already keep blocks apart), and k_regroup splits the groups.
+//   6. k_bwt + k_samples: U in libsais' convention per block and I[k] = 1 + rank of suffix k*r.
+// Inverse, per batch of BWTs L with primary indexes P (the end marker's row in the n+1-row matrix of each block):
+//   1. byte histograms -> C[c] = first row of byte c, as a global row.
 //   2. k_lf_pass: LF(row) = C[c] + occ(c, row) for every row, a stable multisplit ranking with lookback.
 //   3. k_walk_count: from every splitter row (row % D == 0, and P) follow LF to the next splitter, recording the
-//      sublist length; k_rank_jump: synchronous pointer jumping gives each splitter's text position;
-//      k_walk_emit: each sublist is walked again and its bytes written.
+//      sublist length and staging its first bytes; k_rank_jump: synchronous pointer jumping gives each splitter's
+//      text position; k_walk_emit: each sublist's bytes land at their positions.
 #include "kbwt_device.h"
 #include "kbwt_internal.h"
 
 namespace kbwt {
 
+// ------------------------------------------------------------------ tiles and blocks
+
+// The block b with first[b] <= t < first[b + 1].
+__device__ __forceinline__ uint32_t tile_block(const uint32_t *first, uint32_t B, uint32_t t) {
+    uint32_t lo = 0, hi = B;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (first[mid] <= t) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+struct TileRange {
+    uint32_t b;        // block
+    uint32_t tb;       // tile within the block
+    size_t beg, end;   // element range of the tile
+    size_t bend;       // end of the block
+};
+
+__device__ __forceinline__ TileRange tile_range(const Tiling &tl, uint32_t t, uint32_t tile, size_t m) {
+    TileRange r;
+    if (tl.first) {
+        r.b = tile_block(tl.first, tl.B, t);
+        r.tb = t - tl.first[r.b];
+        const BlockInfo bi = tl.blocks[r.b];
+        r.beg = (size_t)bi.off + (size_t)r.tb * tile;
+        r.bend = (size_t)bi.off + bi.n;
+    } else {
+        r.b = 0;
+        r.tb = t;
+        r.beg = (size_t)t * tile;
+        r.bend = m;
+    }
+    r.end = r.beg + tile < r.bend ? r.beg + tile : r.bend;
+    return r;
+}
+
 // ------------------------------------------------------------------ histograms
 //
-// Every histogram kernel accumulates in LDS and adds its non-zero bins to a global `counts` array with one atomic
-// per bin; k_scan_digits turns counts into exclusive scans and clears them again, so counts is all zero between
-// uses (it is zeroed once when allocated).
+// Histogram kernels accumulate in LDS and add their non-zero bins to a global `counts` array, one atomic per bin and
+// block; k_scan_digits turns counts into exclusive scans and clears them again, so counts is all zero between uses.
+// Workgroup g walks a contiguous run of tiles and flushes whenever the block changes.
 
-__global__ __launch_bounds__(256) void k_byte_hist(const uint8_t *__restrict__ T, uint32_t n,
+__global__ __launch_bounds__(256) void k_byte_hist(const uint8_t *__restrict__ T, Tiling tl, uint32_t ntiles,
                                                    uint32_t *__restrict__ counts) {
     __shared__ uint32_t h[4][256];
-    for (int i = threadIdx.x; i < 1024; i += 256) (&h[0][0])[i] = 0;
+    const int tid = threadIdx.x, wave = tid >> 6;
+    for (int i = tid; i < 1024; i += 256) (&h[0][0])[i] = 0;
     __syncthreads();
-    const int wave = threadIdx.x >> 6;
-    const size_t n4 = n / 4;
-    const uint32_t *T4 = reinterpret_cast<const uint32_t *>(T);
-    const size_t stride = (size_t)gridDim.x * 256;
-    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n4; e += stride) {
-        uint32_t x = T4[e];
-        atomicAdd(&h[wave][x & 255u], 1u);
-        atomicAdd(&h[wave][(x >> 8) & 255u], 1u);
-        atomicAdd(&h[wave][(x >> 16) & 255u], 1u);
-        atomicAdd(&h[wave][x >> 24], 1u);
+    const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const uint32_t t0 = blockIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+    int cur = -1;
+    for (uint32_t t = t0; t < t1; t++) {
+        TileRange r = tile_range(tl, t, HIST_TILE, 0);
+        if ((int)r.b != cur) {
+            if (cur >= 0) {
+                __syncthreads();
+                uint32_t c = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
+                if (c) atomicAdd(&counts[(size_t)cur * 256 + tid], c);
+                h[0][tid] = h[1][tid] = h[2][tid] = h[3][tid] = 0;
+                __syncthreads();
+            }
+            cur = (int)r.b;
+        }
+        for (size_t e = r.beg + tid; e < r.end; e += 256) atomicAdd(&h[wave][T[e]], 1u);
     }
-    if (blockIdx.x == 0)
-        for (size_t e = n4 * 4 + threadIdx.x; e < n; e += 256) atomicAdd(&h[wave][T[e]], 1u);
     __syncthreads();
-    uint32_t c = h[0][threadIdx.x] + h[1][threadIdx.x] + h[2][threadIdx.x] + h[3][threadIdx.x];
-    if (c) atomicAdd(&counts[threadIdx.x], c);
+    if (cur >= 0) {
+        uint32_t c = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
+        if (c) atomicAdd(&counts[(size_t)cur * 256 + tid], c);
+    }
 }
 
-// One block per histogram set: tot = counts, excl = base + exclusive scan of counts; counts is cleared.
-__global__ __launch_bounds__(256) void k_scan_digits(uint32_t *__restrict__ counts, uint32_t *__restrict__ tot,
-                                                     uint32_t *__restrict__ excl, uint32_t base) {
+// One workgroup per histogram set (set = block * sets_per_block + p): tot = counts, excl = base + exclusive scan of
+// counts, and counts is cleared. base: 0 (mode 0), the block's first packed index (1), its first row + 1 (2).
+__global__ __launch_bounds__(256) void k_scan_digits(uint32_t *__restrict__ counts, uint32_t sets_per_block,
+                                                     const BlockInfo *__restrict__ blocks, int base_mode,
+                                                     uint32_t *__restrict__ tot, uint32_t *__restrict__ excl) {
     __shared__ uint32_t scratch[8];
-    const uint32_t set = blockIdx.x, b = threadIdx.x;
-    const uint32_t s = counts[set * 256 + b];
+    const uint32_t set = blockIdx.x, d = threadIdx.x;
+    const uint32_t b = set / sets_per_block;
+    const uint32_t base = base_mode == 0 ? 0u : (base_mode == 1 ? blocks[b].off : blocks[b].roff + 1u);
+    const size_t at = (size_t)set * 256 + d;
+    const uint32_t s = counts[at];
     uint32_t total;
     uint32_t ex = block_excl_sum<256>(s, scratch, &total);
-    if (tot) tot[set * 256 + b] = s;
-    if (excl) excl[set * 256 + b] = base + ex;
-    counts[set * 256 + b] = 0;
+    if (tot) tot[at] = s;
+    if (excl) excl[at] = base + ex;
+    counts[at] = 0;
 }
 
-// First key of suffix i: h0 symbol codes of w bits (zero past the end) above the 8-bit BWT byte T[i-1] (0 for i = 0,
-// the primary row, whose byte is never emitted). c points at the code of T[i], raw at T[i-1].
+// First key of suffix i: h0 symbol codes of w bits (zero past the block's end) above the 8-bit BWT byte T[i-1] (0 for
+// the block's first suffix, the primary row, whose byte is never emitted). c points at the code of T[i].
 __device__ __forceinline__ uint64_t build_key(const uint8_t *c, uint8_t bwt_byte, uint32_t h0, uint32_t w) {
     uint64_t k = 0;
     for (uint32_t b = 0; b < h0; b++) k = (k << w) | c[b];
     return (k << 8) | bwt_byte;
 }
 
-// Digit histograms of the first keys, P digits of 8 bits above the BWT byte.
-__global__ __launch_bounds__(256) void k_hist_T(const uint8_t *__restrict__ T, uint32_t n,
-                                                const uint8_t *__restrict__ code, uint32_t w, uint32_t h0,
-                                                uint32_t P, uint32_t *__restrict__ counts) {
-    constexpr int HT = 4096;
+// Digit histograms of the first keys of every block, P digits of 8 bits above the BWT byte: counts[(b*P + p)*256 + d].
+__global__ __launch_bounds__(256) void k_hist_T(const uint8_t *__restrict__ T, Tiling tl, uint32_t ntiles,
+                                                const uint8_t *__restrict__ codes, uint32_t P,
+                                                uint32_t *__restrict__ counts) {
     __shared__ uint32_t h[MAXP][256];
     __shared__ uint8_t s_code[256];
-    __shared__ uint8_t s_c[HT + 64];
-    for (int i = threadIdx.x; i < MAXP * 256; i += 256) (&h[0][0])[i] = 0;
-    s_code[threadIdx.x] = code[threadIdx.x];
-    const uint32_t ntiles = (n + HT - 1) / HT;
-    for (uint32_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const size_t base = (size_t)tile * HT;
+    __shared__ uint8_t s_c[HIST_TILE + 64];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < MAXP * 256; i += 256) (&h[0][0])[i] = 0;
+    const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const uint32_t t0 = blockIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+    int cur = -1;
+    uint32_t w = 0, h0 = 0;
+    for (uint32_t t = t0; t < t1; t++) {
+        TileRange r = tile_range(tl, t, HIST_TILE, 0);
         __syncthreads();
-        for (uint32_t j = threadIdx.x; j < HT + h0; j += 256) {
-            size_t e = base + j;
-            s_c[j] = e < n ? s_code[T[e]] : 0;
+        if ((int)r.b != cur) {
+            if (cur >= 0) {
+                for (uint32_t i = tid; i < P * 256; i += 256) {
+                    uint32_t c = (&h[0][0])[i];
+                    if (c) atomicAdd(&counts[(size_t)cur * P * 256 + i], c);
+                    (&h[0][0])[i] = 0;
+                }
+            }
+            cur = (int)r.b;
+            s_code[tid] = codes[(size_t)r.b * 256 + tid];
+            w = tl.blocks[r.b].w;
+            h0 = tl.blocks[r.b].h0;
+            __syncthreads();
+        }
+        for (uint32_t j = tid; j < HIST_TILE + h0; j += 256) {
+            size_t e = r.beg + j;
+            s_c[j] = e < r.bend ? s_code[T[e]] : 0;
         }
         __syncthreads();
-        for (uint32_t j = threadIdx.x; j < HT; j += 256) {
-            if (base + j >= n) break;
+        for (uint32_t j = tid; j < HIST_TILE; j += 256) {
+            if (r.beg + j >= r.end) break;
             uint64_t key = build_key(s_c + j, 0, h0, w);
             for (uint32_t p = 0; p < P; p++) atomicAdd(&h[p][(uint32_t)(key >> (8 * p + 8)) & 255u], 1u);
         }
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < P * 256; i += 256) {
-        uint32_t c = (&h[0][0])[i];
-        if (c) atomicAdd(&counts[i], c);
-    }
+    if (cur >= 0)
+        for (uint32_t i = tid; i < P * 256; i += 256) {
+            uint32_t c = (&h[0][0])[i];
+            if (c) atomicAdd(&counts[(size_t)cur * P * 256 + i], c);
+        }
 }
 
-// Digit histograms of stored keys at the given shifts.
+// Digit histograms of stored keys at the given shifts (one flat range).
 __global__ __launch_bounds__(256) void k_hist_keys(const uint64_t *__restrict__ K, uint32_t m, Shifts sh,
                                                    uint32_t *__restrict__ counts) {
     __shared__ uint32_t h[MAXP][256];
@@ -126,16 +197,17 @@ __global__ __launch_bounds__(256) void k_hist_keys(const uint64_t *__restrict__ 
 
 // ------------------------------------------------------------------ onesweep radix pass
 
-// Stable pass over m (key, value) pairs on the 8-bit digit at `shift`. FROM_T: keys are built from T (value = the
-// suffix index). dbase[d]: global start of digit d. status: ntiles*256 lookback words; counter: tile-id counter.
-// The tile leaves through one LDS buffer, keys first and then values, so equal digits go out as contiguous runs.
-template <int BLOCK, int ITEMS, bool FROM_T>
+// Stable pass over (key, value) pairs on the 8-bit digit at `shift`, segmented by block when tl.first is set.
+// FROM_T: keys are built from T with the block's code table (value = the suffix index). dbase[b*dstride + d]: start of
+// digit d in block b. status: one lookback word per (tile, digit); counter: tile-id counter.
+template <int BLOCK, int ITEMS, int LB, bool FROM_T>
 __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                      uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                      uint32_t m, uint32_t shift, const uint32_t *__restrict__ dbase,
-                                                     uint64_t *__restrict__ status, uint32_t *__restrict__ counter,
-                                                     uint32_t epoch, uint32_t ntiles, const uint8_t *__restrict__ T,
-                                                     const uint8_t *__restrict__ code, uint32_t w, uint32_t h0) {
+                                                     uint32_t dstride, Tiling tl, uint64_t *__restrict__ status,
+                                                     uint32_t *__restrict__ counter, uint32_t epoch, uint32_t ntiles,
+                                                     const uint8_t *__restrict__ T,
+                                                     const uint8_t *__restrict__ codes) {
     constexpr int W = BLOCK / WAVE;
     constexpr int TILE = BLOCK * ITEMS;
     static_assert(BLOCK >= 256, "one thread per digit");
@@ -151,24 +223,30 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
     if (tid == 0) s_tile = next_tile(counter, ntiles);
     for (int i = tid; i < W * 256; i += BLOCK) (&s_wcnt[0][0])[i] = 0;
-    if (FROM_T && tid < 256) s_code[tid] = code[tid];
     __syncthreads();
     const uint32_t tile = s_tile;
-    const size_t base = (size_t)tile * TILE;
+    const TileRange tr = tile_range(tl, tile, TILE, m);
+    const size_t base = tr.beg;
+    const uint32_t cnt = (uint32_t)(tr.end - tr.beg);
+    const uint32_t *db = dbase + (size_t)tr.b * dstride;
 
     uint64_t key[ITEMS];
     uint32_t val[ITEMS];
     if (FROM_T) {
-        // raw[j] = T[base + j - 1] (the BWT byte of element j), cod[j] = code of T[base + j], zero past the end
+        // raw[j] = T[base + j - 1] (the BWT byte of element j), cod[j] = code of T[base + j], zero past the block
+        const uint32_t w = tl.blocks[tr.b].w, h0 = tl.blocks[tr.b].h0;
+        const size_t boff = tl.blocks[tr.b].off;
+        if (tid < 256) s_code[tid] = codes[(size_t)tr.b * 256 + tid];
         uint8_t *s_raw = reinterpret_cast<uint8_t *>(s_buf);
         uint8_t *s_cod = s_raw + TILE + 64;
+        __syncthreads();
         for (uint32_t j = tid; j < (uint32_t)TILE + h0; j += BLOCK) {
             size_t e = base + j;
-            uint8_t t = e < m ? T[e] : 0;
-            s_cod[j] = e < m ? s_code[t] : 0;
+            uint8_t t = e < tr.bend ? T[e] : 0;
+            s_cod[j] = e < tr.bend ? s_code[t] : 0;
             if (j + 1 <= (uint32_t)TILE) s_raw[j + 1] = t;
         }
-        if (tid == 0) s_raw[0] = base > 0 ? T[base - 1] : 0;
+        if (tid == 0) s_raw[0] = base > boff ? T[base - 1] : 0;
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
@@ -180,8 +258,9 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     } else {
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
-            size_t e = base + wave * WAVE * ITEMS + k * WAVE + lane;
-            if (e < m) { key[k] = kin[e]; val[k] = vin[e]; } else { key[k] = 0; val[k] = 0; }
+            uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
+            if (local < cnt) { key[k] = kin[base + local]; val[k] = vin[base + local]; }
+            else { key[k] = 0; val[k] = 0; }
         }
     }
 
@@ -189,8 +268,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     uint32_t rk[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        size_t e = base + wave * WAVE * ITEMS + k * WAVE + lane;
-        bool valid = e < m;
+        uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
+        bool valid = local < cnt;
         uint32_t d = (uint32_t)(key[k] >> shift) & 255u;
         uint64_t peers = match_digit8(d, __ballot(valid));
         rk[k] = 0;
@@ -203,38 +282,36 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     }
     __syncthreads();
 
-    // per digit: wave prefixes, tile total, lookback over earlier tiles
+    // per digit: wave prefixes, tile total, lookback over earlier tiles of the block
     uint32_t tot = 0;
     if (tid < 256) {
 #pragma unroll
         for (int w2 = 0; w2 < W; w2++) { uint32_t c = s_wcnt[w2][tid]; s_wcnt[w2][tid] = tot; tot += c; }
-        if (tile == 0) st_store(status + tid, st_pack(epoch, ST_INC, tot));
-        else st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, ST_AGG, tot));
+        st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, tr.tb == 0 ? ST_INC : ST_AGG, tot));
     }
     uint32_t tstart = block_excl_sum<BLOCK>(tid < 256 ? tot : 0u, s_scratch, nullptr);
     if (tid < 256) {
         uint32_t excl = 0;
-        if (tile > 0) {
-            excl = lookback<false>(status + tid, 256, tile, epoch);
+        if (tr.tb > 0) {
+            excl = lookback_win<LB>(status + tid, 256, tile, epoch);
             st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, ST_INC, excl + tot));
         }
         s_tstart[tid] = tstart;
-        s_gbase[tid] = dbase[tid] + excl - tstart;
+        s_gbase[tid] = db[tid] + excl - tstart;
     }
     __syncthreads();
 
-    // local sorted position of each item
+    // local sorted position of each item; keys leave first, then values, through one LDS buffer
     uint32_t lidx[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         uint32_t d = (uint32_t)(key[k] >> shift) & 255u;
         lidx[k] = s_tstart[d] + s_wcnt[wave][d] + rk[k];
     }
-    const uint32_t cnt = (uint32_t)((m - base) < (size_t)TILE ? (m - base) : (size_t)TILE);
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        size_t e = base + wave * WAVE * ITEMS + k * WAVE + lane;
-        if (e < m) s_buf[lidx[k]] = key[k];
+        uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
+        if (local < cnt) s_buf[lidx[k]] = key[k];
     }
     __syncthreads();
     uint32_t dg[ITEMS];
@@ -252,8 +329,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     uint32_t *s_v = reinterpret_cast<uint32_t *>(s_buf);
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        size_t e = base + wave * WAVE * ITEMS + k * WAVE + lane;
-        if (e < m) s_v[lidx[k]] = val[k];
+        uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
+        if (local < cnt) s_v[lidx[k]] = val[k];
     }
     __syncthreads();
 #pragma unroll
@@ -266,19 +343,20 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
 // ------------------------------------------------------------------ regroup (group heads, ISA, compaction)
 
 // Sorted keys K[0..m) with values V (suffixes). Row a goes to SA position pos[a] (pos == nullptr: a). Rows a-1 and a
-// share a group when their keys agree above bit kshift. Writes SA[pos[a]] = V[a] (when SA != nullptr), ISA[V[a]] =
-// head of its group (SA position of the group's first row), the BWT byte of row pos[a] into L (the key's low byte
-// when T == nullptr, else T[V[a]-1]), and compacts the rows of groups of two or more into (pos_out, grp_out); the
-// last tile writes their count to *m_out.
+// share a group when their keys agree above bit kshift (and, with blk, lie in the same block). Writes SA[pos[a]] =
+// V[a] (when SA != nullptr), ISA[V[a]] = head of its group (SA position of the group's first row), the BWT byte of
+// row pos[a] into L (the key's low byte when T == nullptr, else T[V[a]-1]; the byte of a block's first suffix is
+// never used), and compacts the rows of groups of two or more into (pos_out, grp_out); the last tile writes their
+// count to *m_out.
 template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ K, const uint32_t *__restrict__ V,
                                                    const uint32_t *__restrict__ pos, uint32_t m, uint32_t kshift,
-                                                   uint32_t *__restrict__ SA, uint32_t *__restrict__ ISA,
-                                                   uint8_t *__restrict__ L, const uint8_t *__restrict__ T,
-                                                   uint32_t *__restrict__ pos_out, uint32_t *__restrict__ grp_out,
-                                                   uint32_t *__restrict__ m_out, uint64_t *__restrict__ st_sum,
-                                                   uint64_t *__restrict__ st_max, uint32_t *__restrict__ counter,
-                                                   uint32_t epoch, uint32_t ntiles) {
+                                                   const uint16_t *__restrict__ blk, uint32_t *__restrict__ SA,
+                                                   uint32_t *__restrict__ ISA, uint8_t *__restrict__ L,
+                                                   const uint8_t *__restrict__ T, uint32_t *__restrict__ pos_out,
+                                                   uint32_t *__restrict__ grp_out, uint32_t *__restrict__ m_out,
+                                                   uint64_t *__restrict__ st_sum, uint64_t *__restrict__ st_max,
+                                                   uint32_t *__restrict__ counter, uint32_t epoch, uint32_t ntiles) {
     constexpr int TILE = BLOCK * ITEMS;
     __shared__ uint32_t s_scratch[32];
     __shared__ uint32_t s_tile, s_ex_sum, s_ex_max;
@@ -304,6 +382,11 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
             uint64_t kc = kr >> kshift;
             flag = (a == 0) || ((K[a - 1] >> kshift) != kc);
             bool flagn = (a + 1 >= m) || ((K[a + 1] >> kshift) != kc);
+            if (blk) {
+                uint16_t bc = blk[a];
+                flag = flag || (a > 0 && blk[a - 1] != bc);
+                flagn = flagn || (a + 1 < m && blk[a + 1] != bc);
+            }
             keep = !(flag && flagn);
             p = pos ? pos[a] : (uint32_t)a;
             lbyte[k] = (uint8_t)kr;
@@ -317,23 +400,32 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
         run_sum += tot_s;
         run_max = run_max > tot_m ? run_max : tot_m;
     }
-    if (tid == 0) {
-        uint32_t ex_s = 0, ex_m = 0;
+    // wave 0 looks back over the compaction counts, wave 1 over the group heads
+    const int wave = tid / WAVE;
+    if (wave == 0) {
+        uint32_t ex_s = 0;
         if (tile == 0) {
-            st_store(st_sum, st_pack(epoch, ST_INC, run_sum));
-            st_store(st_max, st_pack(epoch, ST_INC, run_max));
+            if (tid == 0) st_store(st_sum, st_pack(epoch, ST_INC, run_sum));
         } else {
-            st_store(st_sum + tile, st_pack(epoch, ST_AGG, run_sum));
-            st_store(st_max + tile, st_pack(epoch, ST_AGG, run_max));
-            ex_s = lookback<false>(st_sum, 1, tile, epoch);
-            ex_m = lookback<true>(st_max, 1, tile, epoch);
-            st_store(st_sum + tile, st_pack(epoch, ST_INC, ex_s + run_sum));
-            uint32_t im = ex_m > run_max ? ex_m : run_max;
-            st_store(st_max + tile, st_pack(epoch, ST_INC, im));
+            if (tid == 0) st_store(st_sum + tile, st_pack(epoch, ST_AGG, run_sum));
+            ex_s = lookback_wave<false>(st_sum, tile, epoch);
+            if (tid == 0) st_store(st_sum + tile, st_pack(epoch, ST_INC, ex_s + run_sum));
         }
-        s_ex_sum = ex_s;
-        s_ex_max = ex_m;
-        if (tile == ntiles - 1) *m_out = ex_s + run_sum;
+        if (tid == 0) {
+            s_ex_sum = ex_s;
+            if (tile == ntiles - 1) *m_out = ex_s + run_sum;
+        }
+    } else if (wave == 1) {
+        uint32_t ex_m = 0;
+        if (tile == 0) {
+            if (tid == WAVE) st_store(st_max, st_pack(epoch, ST_INC, run_max));
+        } else {
+            if (tid == WAVE) st_store(st_max + tile, st_pack(epoch, ST_AGG, run_max));
+            ex_m = lookback_wave<true>(st_max, tile, epoch);
+            uint32_t im = ex_m > run_max ? ex_m : run_max;
+            if (tid == WAVE) st_store(st_max + tile, st_pack(epoch, ST_INC, im));
+        }
+        if (tid == WAVE) s_ex_max = ex_m;
     }
     __syncthreads();
     const uint32_t ex_sum = s_ex_sum, ex_max = s_ex_max;
@@ -357,20 +449,25 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
 
 // ------------------------------------------------------------------ doubling round keys
 
-// key = (group head << 32) | sec, sec = ISA[i+h] + h when i+h < n, else n-1-i (below every rank, shorter suffix
-// first). orand[0] |= keys, orand[1] &= keys (which digits vary): one pair of atomics per workgroup.
+// key = (group head << 32) | sec, sec = ISA[i+h] + h when i+h lies inside the block, else end-1-i (below every rank,
+// shorter suffix first). orand[0] |= keys, orand[1] &= keys (which digits vary): one pair of atomics per workgroup.
 __global__ __launch_bounds__(256) void k_round_keys(const uint32_t *__restrict__ pos, const uint32_t *__restrict__ grp,
                                                     uint32_t m, const uint32_t *__restrict__ SA,
-                                                    const uint32_t *__restrict__ ISA, uint32_t n, uint32_t h,
+                                                    const uint32_t *__restrict__ ISA, uint32_t n,
+                                                    const uint16_t *__restrict__ blk,
+                                                    const BlockInfo *__restrict__ blocks, uint32_t h,
                                                     uint64_t *__restrict__ K, uint32_t *__restrict__ V,
                                                     unsigned long long *__restrict__ orand) {
     __shared__ uint64_t s_o[4], s_a[4];
     uint64_t ko = 0, ka = ~0ull;
     const size_t stride = (size_t)gridDim.x * 256;
     for (size_t a = (size_t)blockIdx.x * 256 + threadIdx.x; a < m; a += stride) {
-        uint32_t i = SA[pos[a]];
+        uint32_t p = pos[a];
+        uint32_t i = SA[p];
+        uint32_t end = n;
+        if (blk) { const BlockInfo bi = blocks[blk[p]]; end = bi.off + bi.n; }
         uint64_t j = (uint64_t)i + h;
-        uint32_t sec = j < n ? ISA[j] + h : n - 1u - i;
+        uint32_t sec = j < end ? ISA[j] + h : end - 1u - i;
         uint64_t key = ((uint64_t)grp[a] << 32) | sec;
         K[a] = key;
         V[a] = i;
@@ -394,31 +491,53 @@ __global__ __launch_bounds__(256) void k_round_keys(const uint32_t *__restrict__
 
 // ------------------------------------------------------------------ BWT emission
 
-// U in libsais' convention from the row bytes L[p] = T[SA[p]-1]: U[0] = T[n-1], row p goes to U[p+1] before the
-// primary row p0 = ISA[0] and to U[p] after it (row p0 itself, the end marker's, is dropped). Four bytes per thread.
+// U of every block in libsais' convention from the row bytes L[p] = T[SA[p]-1]: U[0] = T[n-1], row p goes to U[p+1]
+// before the primary row p0 (the rank of the block's first suffix) and to U[p] after it. Four bytes per thread.
 __global__ __launch_bounds__(256) void k_bwt(const uint8_t *__restrict__ T, const uint8_t *__restrict__ L,
-                                             const uint32_t *__restrict__ ISA, uint8_t *__restrict__ U, uint32_t n) {
-    const uint32_t p0 = ISA[0];
+                                             const uint32_t *__restrict__ ISA, const uint16_t *__restrict__ blk,
+                                             const BlockInfo *__restrict__ blocks, uint8_t *__restrict__ U,
+                                             uint32_t N) {
     size_t o0 = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        size_t o = o0 + k;   // output byte o comes from row o-1 (o <= p0) or row o (o > p0)
-        if (o >= n) break;
-        U[o] = o == 0 ? T[n - 1] : (o <= p0 ? L[o - 1] : L[o]);
+        size_t o = o0 + k;
+        if (o >= N) break;
+        uint32_t off = 0, n = N;
+        if (blk) { const BlockInfo bi = blocks[blk[o]]; off = bi.off; n = bi.n; }
+        const uint32_t p0 = ISA[off] - off;
+        const size_t lo = o - off;   // output byte lo of the block comes from row lo-1 (lo <= p0) or row lo (lo > p0)
+        U[o] = lo == 0 ? T[off + n - 1] : (lo <= p0 ? L[o - 1] : L[o]);
     }
 }
 
-__global__ void k_samples(const uint32_t *__restrict__ ISA, uint32_t r, uint32_t count, int32_t *__restrict__ I) {
-    size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (k < count) I[k] = (int32_t)(ISA[k * (size_t)r] + 1u);
+// I[g] for the g-th sampled index of the batch: block b with soff <= g, k = g - soff, I = 1 + rank of suffix k*r.
+__global__ __launch_bounds__(256) void k_samples(const uint32_t *__restrict__ ISA, const BlockInfo *__restrict__ blocks,
+                                                 uint32_t B, uint32_t count, int32_t *__restrict__ I) {
+    size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= count) return;
+    uint32_t lo = 0, hi = B;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (blocks[mid].soff <= g) lo = mid; else hi = mid;
+    }
+    const BlockInfo bi = blocks[lo];
+    const size_t k = g - bi.soff;
+    I[g] = (int32_t)(ISA[bi.off + k * bi.r] - bi.off + 1u);
+}
+
+// blk[o] = the block of packed index o (grid.y = block).
+__global__ __launch_bounds__(256) void k_fill_blk(uint16_t *__restrict__ blk, const BlockInfo *__restrict__ blocks) {
+    const BlockInfo bi = blocks[blockIdx.y];
+    for (size_t o = (size_t)blockIdx.x * 256 + threadIdx.x; o < bi.n; o += (size_t)gridDim.x * 256)
+        blk[bi.off + o] = (uint16_t)blockIdx.y;
 }
 
 // ------------------------------------------------------------------ inverse BWT
 
-// LF for every byte of the BWT L[0..n): lf[q] = C[L[q]] + #{q' < q : L[q'] = L[q]}. The row of L[q] in the full
-// matrix is q before the primary index, q+1 from it on, so lf[q] is LF of that row. cbase[c] = C[c].
+// LF for every byte of every block's BWT: lf[q] = C_b[L[q]] + #{q' < q in the block : L[q'] = L[q]}, a global row.
+// The row of L[q] in the block's full matrix is q before the primary index and q + 1 from it on.
 template <int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void k_lf_pass(const uint8_t *__restrict__ L, uint32_t n,
+__global__ __launch_bounds__(BLOCK) void k_lf_pass(const uint8_t *__restrict__ L, Tiling tl,
                                                    const uint32_t *__restrict__ cbase, uint32_t *__restrict__ lf,
                                                    uint64_t *__restrict__ status, uint32_t *__restrict__ counter,
                                                    uint32_t epoch, uint32_t ntiles) {
@@ -432,13 +551,14 @@ __global__ __launch_bounds__(BLOCK) void k_lf_pass(const uint8_t *__restrict__ L
     for (int i = tid; i < W * 256; i += BLOCK) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const size_t base = (size_t)tile * TILE;
+    const TileRange tr = tile_range(tl, tile, TILE, 0);
+    const uint32_t cnt = (uint32_t)(tr.end - tr.beg);
     uint32_t dg[ITEMS], rk[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        size_t e = base + wave * WAVE * ITEMS + k * WAVE + lane;
-        bool valid = e < n;
-        uint32_t d = valid ? L[e] : 0u;
+        uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
+        bool valid = local < cnt;
+        uint32_t d = valid ? L[tr.beg + local] : 0u;
         uint64_t peers = match_digit8(d, __ballot(valid));
         dg[k] = d;
         rk[k] = 0;
@@ -455,23 +575,23 @@ __global__ __launch_bounds__(BLOCK) void k_lf_pass(const uint8_t *__restrict__ L
 #pragma unroll
         for (int w2 = 0; w2 < W; w2++) { uint32_t c = s_wcnt[w2][tid]; s_wcnt[w2][tid] = tot; tot += c; }
         uint32_t excl = 0;
-        if (tile == 0) st_store(status + tid, st_pack(epoch, ST_INC, tot));
+        if (tr.tb == 0) st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, ST_INC, tot));
         else {
             st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, ST_AGG, tot));
-            excl = lookback<false>(status + tid, 256, tile, epoch);
+            excl = lookback_win<4>(status + tid, 256, tile, epoch);
             st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, ST_INC, excl + tot));
         }
-        s_base[tid] = cbase[tid] + excl;
+        s_base[tid] = cbase[(size_t)tr.b * 256 + tid] + excl;
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        size_t e = base + wave * WAVE * ITEMS + k * WAVE + lane;
-        if (e < n) lf[e] = s_base[dg[k]] + s_wcnt[wave][dg[k]] + rk[k];
+        uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
+        if (local < cnt) lf[tr.beg + local] = s_base[dg[k]] + s_wcnt[wave][dg[k]] + rk[k];
     }
 }
 
-// Byte of a row from its LF value: the c with C[c] <= v < C[c+1] (s_c holds C[0..255]).
+// Byte of a row from its LF value (a global row): the c with C[c] <= v < C[c+1] (s_c holds the block's C[0..255]).
 __device__ __forceinline__ uint8_t row_byte(const uint32_t *s_c, uint32_t v) {
     uint32_t lo = 0;
 #pragma unroll
@@ -480,38 +600,44 @@ __device__ __forceinline__ uint8_t row_byte(const uint32_t *s_c, uint32_t v) {
     return (uint8_t)lo;
 }
 
-// Splitter s: row s*D for s < nreg, row P for s == nreg (only when P % D != 0). Walk LF from it to the next splitter
-// row; link[s] = (steps << 32) | next splitter, len[s] = steps. The bytes met on the way (the text backwards from the
-// splitter's suffix) go to stage[s*cap ..] for the first cap steps; cont[s] keeps the row of step cap of a longer
-// sublist. The marker row P ends the list: link = (0, S).
-__global__ __launch_bounds__(256) void k_walk_count(const uint32_t *__restrict__ lf, uint32_t n, uint32_t P,
-                                                    uint32_t dshift, uint32_t nreg, uint32_t S,
-                                                    const uint32_t *__restrict__ cbase, uint64_t *__restrict__ link,
-                                                    uint32_t *__restrict__ len, uint8_t *__restrict__ stage,
-                                                    uint32_t cap, uint32_t *__restrict__ cont) {
+// Splitters of block b: local j < nreg is the row j*D, j == nreg the row P (only when P % D != 0); global id
+// spoff + j. From each, walk LF to the next splitter row of the block; link = (steps << 32) | next splitter, len =
+// steps. The bytes met on the way (the text backwards from the splitter's suffix) go to stage[id*cap ..] for the
+// first cap steps; cont keeps the local row of step cap of a longer sublist. The marker row P ends the list:
+// link = (0, S). Workgroups take WALK_TILE splitters of one block.
+__global__ __launch_bounds__(WALK_TILE) void k_walk_count(const uint32_t *__restrict__ lf, Tiling tl, uint32_t dshift,
+                                                          uint32_t S, const uint32_t *__restrict__ cbase,
+                                                          uint64_t *__restrict__ link, uint32_t *__restrict__ len,
+                                                          uint8_t *__restrict__ stage, uint32_t cap,
+                                                          uint32_t *__restrict__ cont) {
     __shared__ uint32_t s_c[256];
-    s_c[threadIdx.x] = cbase[threadIdx.x];
+    const uint32_t b = tile_block(tl.first, tl.B, blockIdx.x);
+    const uint32_t tb = blockIdx.x - tl.first[b];
+    const BlockInfo bi = tl.blocks[b];
+    s_c[threadIdx.x] = cbase[(size_t)b * 256 + threadIdx.x];
     __syncthreads();
-    size_t s = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (s > S) return;
-    if (s == S) { link[S] = S; return; }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) link[S] = S;
     const uint32_t dmask = (1u << dshift) - 1u;
-    const uint32_t sidP = (P & dmask) == 0 ? (P >> dshift) : nreg;
-    uint32_t row = s < nreg ? (uint32_t)(s << dshift) : P;
-    if (row == P) { link[s] = S; len[s] = 0; return; }
+    const uint32_t nsp = bi.nreg + ((bi.P & dmask) != 0 ? 1u : 0u);
+    const uint32_t j = tb * WALK_TILE + threadIdx.x;
+    if (j >= nsp) return;
+    const uint32_t id = bi.spoff + j;
+    const uint32_t sidP = bi.spoff + ((bi.P & dmask) == 0 ? (bi.P >> dshift) : bi.nreg);
+    uint32_t r = j < bi.nreg ? (j << dshift) : bi.P;
+    if (r == bi.P) { link[id] = S; len[id] = 0; return; }
     uint32_t steps = 0;
-    uint8_t *st = stage + s * (size_t)cap;
+    uint8_t *st = stage + (size_t)id * cap;
     do {
-        if (steps == cap) cont[s] = row;
-        uint32_t q = row < P ? row : row - 1u;
+        if (steps == cap) cont[id] = r;
+        uint32_t q = bi.off + (r < bi.P ? r : r - 1u);
         uint32_t v = lf[q];
         if (steps < cap) st[steps] = row_byte(s_c, v);
         steps++;
-        row = v;
-    } while ((row & dmask) != 0 && row != P && steps <= n);
-    uint32_t nxt = row == P ? sidP : (row >> dshift);
-    link[s] = ((uint64_t)steps << 32) | nxt;
-    len[s] = steps;
+        r = v - bi.roff;
+    } while ((r & dmask) != 0 && r != bi.P && steps <= bi.n);
+    uint32_t nxt = r == bi.P ? sidP : bi.spoff + (r >> dshift);
+    link[id] = ((uint64_t)steps << 32) | nxt;
+    len[id] = steps;
 }
 
 // One synchronous pointer-jumping step: out[s] = (acc[s] + acc[ptr], ptr[ptr]) unless ptr == S (the end).
@@ -528,35 +654,61 @@ __global__ __launch_bounds__(256) void k_rank_jump(const uint64_t *__restrict__ 
     out[s] = v;
 }
 
-// Write every sublist: its splitter's suffix starts at text position pos = link >> 32 (after ranking), and step t of
-// the walk is the byte T[pos-1-t]. The first cap bytes come from stage; a longer sublist walks on from cont[s].
-__global__ __launch_bounds__(256) void k_walk_emit(const uint32_t *__restrict__ lf, uint32_t n, uint32_t P,
-                                                   uint32_t S, const uint64_t *__restrict__ link,
-                                                   const uint32_t *__restrict__ len, const uint32_t *__restrict__ cbase,
-                                                   const uint8_t *__restrict__ stage, uint32_t cap,
-                                                   const uint32_t *__restrict__ cont, uint8_t *__restrict__ out) {
+// A block's BWT is valid when the list from its row 0 (splitter spoff) reaches the end with n steps.
+__global__ __launch_bounds__(256) void k_check_lists(const uint64_t *__restrict__ link,
+                                                     const BlockInfo *__restrict__ blocks, uint32_t B, uint32_t S,
+                                                     uint32_t *__restrict__ bad) {
+    size_t b = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (b >= B) return;
+    const uint64_t l = link[blocks[b].spoff];
+    bad[b] = ((uint32_t)l == S && (uint32_t)(l >> 32) == blocks[b].n) ? 0u : 1u;
+}
+
+// Write every sublist: its splitter's suffix starts at block position pos = link >> 32 (after ranking), and step t of
+// the walk is the byte at pos-1-t. The first cap bytes come from stage; a longer sublist walks on from cont.
+__global__ __launch_bounds__(WALK_TILE) void k_walk_emit(const uint32_t *__restrict__ lf, Tiling tl, uint32_t dshift,
+                                                         uint32_t S, const uint64_t *__restrict__ link,
+                                                         const uint32_t *__restrict__ len,
+                                                         const uint32_t *__restrict__ cbase,
+                                                         const uint8_t *__restrict__ stage, uint32_t cap,
+                                                         const uint32_t *__restrict__ cont,
+                                                         const uint32_t *__restrict__ bad, uint8_t *__restrict__ out) {
     __shared__ uint32_t s_c[256];
-    s_c[threadIdx.x] = cbase[threadIdx.x];
+    const uint32_t b = tile_block(tl.first, tl.B, blockIdx.x);
+    const uint32_t tb = blockIdx.x - tl.first[b];
+    const BlockInfo bi = tl.blocks[b];
+    s_c[threadIdx.x] = cbase[(size_t)b * 256 + threadIdx.x];
     __syncthreads();
-    size_t s = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (s >= S) return;
-    const uint32_t steps = len[s];
+    if (bad[b]) return;
+    const uint32_t dmask = (1u << dshift) - 1u;
+    const uint32_t nsp = bi.nreg + ((bi.P & dmask) != 0 ? 1u : 0u);
+    const uint32_t j = tb * WALK_TILE + threadIdx.x;
+    if (j >= nsp) return;
+    const uint32_t id = bi.spoff + j;
+    const uint32_t steps = len[id];
     if (steps == 0) return;
-    const uint32_t pos = (uint32_t)(link[s] >> 32);
-    if (pos > n || steps > pos) return;   // only for a string that is not a BWT; the caller rejects those first
-    const uint8_t *st = stage + s * (size_t)cap;
+    const uint32_t pos = (uint32_t)(link[id] >> 32);
+    if (pos > bi.n || steps > pos) return;
+    const uint8_t *st = stage + (size_t)id * cap;
     const uint32_t staged = steps < cap ? steps : cap;
-    uint8_t *o = out + pos - 1;
+    uint8_t *o = out + bi.off + pos - 1;
     for (uint32_t t = 0; t < staged; t++) o[-(int64_t)t] = st[t];
     if (steps > cap) {
-        uint32_t row = cont[s];
+        uint32_t r = cont[id];
         for (uint32_t t = cap; t < steps; t++) {
-            uint32_t q = row < P ? row : row - 1u;
+            uint32_t q = bi.off + (r < bi.P ? r : r - 1u);
             uint32_t v = lf[q];
             o[-(int64_t)t] = row_byte(s_c, v);
-            row = v;
+            r = v - bi.roff;
         }
     }
+}
+
+// Blocks whose string is no BWT decode to zeros (the caller's checksum rejects them).
+__global__ __launch_bounds__(256) void k_zero_bad(uint8_t *__restrict__ out, const uint16_t *__restrict__ blk,
+                                                  const uint32_t *__restrict__ bad, uint32_t N) {
+    size_t o = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (o < N && bad[blk ? blk[o] : 0]) out[o] = 0;
 }
 
 __global__ void k_fill_u8(uint8_t *__restrict__ p, uint8_t v, size_t n) {
@@ -571,67 +723,103 @@ __global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const u
 
 // ------------------------------------------------------------------ launch wrappers
 
-void launch_byte_hist(const uint8_t *T, uint32_t n, uint32_t *counts, uint32_t grid, hipStream_t s) {
-    k_byte_hist<<<grid, 256, 0, s>>>(T, n, counts);
+void launch_byte_hist(const uint8_t *T, Tiling tl, uint32_t ntiles, uint32_t *counts, hipStream_t s) {
+    const uint32_t grid = ntiles < (uint32_t)HIST_GRID ? ntiles : (uint32_t)HIST_GRID;
+    if (grid) k_byte_hist<<<grid, 256, 0, s>>>(T, tl, ntiles, counts);
 }
-void launch_scan_digits(uint32_t *counts, uint32_t nsets, uint32_t *tot, uint32_t *excl, uint32_t base,
-                        hipStream_t s) {
-    k_scan_digits<<<nsets, 256, 0, s>>>(counts, tot, excl, base);
+void launch_scan_digits(uint32_t *counts, uint32_t nsets, uint32_t sets_per_block, const BlockInfo *blocks,
+                        int base_mode, uint32_t *tot, uint32_t *excl, hipStream_t s) {
+    if (nsets) k_scan_digits<<<nsets, 256, 0, s>>>(counts, sets_per_block, blocks, base_mode, tot, excl);
 }
-void launch_hist_T(const uint8_t *T, uint32_t n, const uint8_t *code, uint32_t w, uint32_t h0, uint32_t P,
-                   uint32_t *counts, uint32_t grid, hipStream_t s) {
-    k_hist_T<<<grid, 256, 0, s>>>(T, n, code, w, h0, P, counts);
+void launch_hist_T(const uint8_t *T, Tiling tl, uint32_t ntiles, const uint8_t *codes, uint32_t P, uint32_t *counts,
+                   hipStream_t s) {
+    const uint32_t grid = ntiles < (uint32_t)HIST_GRID ? ntiles : (uint32_t)HIST_GRID;
+    if (grid) k_hist_T<<<grid, 256, 0, s>>>(T, tl, ntiles, codes, P, counts);
 }
 void launch_hist_keys(const uint64_t *K, uint32_t m, const Shifts &sh, uint32_t *counts, uint32_t grid,
                       hipStream_t s) {
     k_hist_keys<<<grid, 256, 0, s>>>(K, m, sh, counts);
 }
-void launch_sort_pass(const SortArgs &a, hipStream_t s) {
-    const uint32_t ntiles = (a.m + SORT_TILE - 1) / SORT_TILE;
+
+// The radix pass configurations the tuner chooses from: (workgroup size, items per thread, lookback window).
+const SortCfg SORT_CFGS[SORT_NCFG] = {{512, 16, 4}, {256, 16, 4}, {256, 16, 1}, {512, 8, 4},
+                                      {256, 12, 4}, {256, 8, 4}, {512, 16, 1}};
+uint32_t sort_tile(int cfg) { return (uint32_t)(SORT_CFGS[cfg].block * SORT_CFGS[cfg].items); }
+
+template <int BLOCK, int ITEMS, int LB>
+static void sort_pass_cfg(const SortArgs &a, uint32_t ntiles, hipStream_t s) {
     if (a.from_t)
-        k_sort_pass<SORT_BLOCK, SORT_ITEMS, true><<<ntiles, SORT_BLOCK, 0, s>>>(
-            a.kin, a.vin, a.kout, a.vout, a.m, a.shift, a.dbase, a.status, a.counter, a.epoch, ntiles, a.T, a.code,
-            a.w, a.h0);
+        k_sort_pass<BLOCK, ITEMS, LB, true><<<ntiles, BLOCK, 0, s>>>(a.kin, a.vin, a.kout, a.vout, a.m, a.shift, a.dbase,
+                                                                    a.dstride, a.tiling, a.status, a.counter, a.epoch,
+                                                                    ntiles, a.T, a.codes);
     else
-        k_sort_pass<SORT_BLOCK, SORT_ITEMS, false><<<ntiles, SORT_BLOCK, 0, s>>>(
-            a.kin, a.vin, a.kout, a.vout, a.m, a.shift, a.dbase, a.status, a.counter, a.epoch, ntiles, a.T, a.code,
-            a.w, a.h0);
+        k_sort_pass<BLOCK, ITEMS, LB, false><<<ntiles, BLOCK, 0, s>>>(a.kin, a.vin, a.kout, a.vout, a.m, a.shift,
+                                                                     a.dbase, a.dstride, a.tiling, a.status,
+                                                                     a.counter, a.epoch, ntiles, a.T, a.codes);
+}
+void launch_sort_pass(const SortArgs &a, uint32_t ntiles, hipStream_t s) {
+    if (!ntiles) return;
+    switch (a.cfg) {
+    case 1: sort_pass_cfg<256, 16, 4>(a, ntiles, s); break;
+    case 2: sort_pass_cfg<256, 16, 1>(a, ntiles, s); break;
+    case 3: sort_pass_cfg<512, 8, 4>(a, ntiles, s); break;
+    case 4: sort_pass_cfg<256, 12, 4>(a, ntiles, s); break;
+    case 5: sort_pass_cfg<256, 8, 4>(a, ntiles, s); break;
+    case 6: sort_pass_cfg<512, 16, 1>(a, ntiles, s); break;
+    default: sort_pass_cfg<512, 16, 4>(a, ntiles, s); break;
+    }
 }
 void launch_regroup(const RegroupArgs &a, hipStream_t s) {
     const uint32_t ntiles = (a.m + RG_TILE - 1) / RG_TILE;
-    k_regroup<RG_BLOCK, RG_ITEMS><<<ntiles, RG_BLOCK, 0, s>>>(a.K, a.V, a.pos, a.m, a.kshift, a.SA, a.ISA, a.L, a.T,
-                                                             a.pos_out, a.grp_out, a.m_out, a.st_sum, a.st_max,
+    if (!ntiles) return;
+    k_regroup<RG_BLOCK, RG_ITEMS><<<ntiles, RG_BLOCK, 0, s>>>(a.K, a.V, a.pos, a.m, a.kshift, a.blk, a.SA, a.ISA, a.L,
+                                                             a.T, a.pos_out, a.grp_out, a.m_out, a.st_sum, a.st_max,
                                                              a.counter, a.epoch, ntiles);
 }
 void launch_round_keys(const uint32_t *pos, const uint32_t *grp, uint32_t m, const uint32_t *SA, const uint32_t *ISA,
-                       uint32_t n, uint32_t h, uint64_t *K, uint32_t *V, unsigned long long *orand, hipStream_t s) {
+                       uint32_t n, const uint16_t *blk, const BlockInfo *blocks, uint32_t h, uint64_t *K, uint32_t *V,
+                       unsigned long long *orand, hipStream_t s) {
     uint32_t grid = (m + 255) / 256;
     if (grid > 2048) grid = 2048;
-    k_round_keys<<<grid, 256, 0, s>>>(pos, grp, m, SA, ISA, n, h, K, V, orand);
+    if (grid) k_round_keys<<<grid, 256, 0, s>>>(pos, grp, m, SA, ISA, n, blk, blocks, h, K, V, orand);
 }
-void launch_bwt(const uint8_t *T, const uint8_t *L, const uint32_t *ISA, uint8_t *U, uint32_t n, hipStream_t s) {
-    k_bwt<<<(n + 1023) / 1024, 256, 0, s>>>(T, L, ISA, U, n);
+void launch_bwt(const uint8_t *T, const uint8_t *L, const uint32_t *ISA, const uint16_t *blk, const BlockInfo *blocks,
+                uint8_t *U, uint32_t N, hipStream_t s) {
+    if (N) k_bwt<<<(N + 1023) / 1024, 256, 0, s>>>(T, L, ISA, blk, blocks, U, N);
 }
-void launch_samples(const uint32_t *ISA, uint32_t r, uint32_t count, int32_t *I, hipStream_t s) {
-    k_samples<<<(count + 255) / 256, 256, 0, s>>>(ISA, r, count, I);
+void launch_samples(const uint32_t *ISA, const BlockInfo *blocks, uint32_t B, uint32_t count, int32_t *I,
+                    hipStream_t s) {
+    if (count) k_samples<<<(count + 255) / 256, 256, 0, s>>>(ISA, blocks, B, count, I);
 }
-void launch_lf_pass(const uint8_t *L, uint32_t n, const uint32_t *cbase, uint32_t *lf, uint64_t *status,
-                    uint32_t *counter, uint32_t epoch, hipStream_t s) {
-    const uint32_t ntiles = (n + LF_TILE - 1) / LF_TILE;
-    k_lf_pass<LF_BLOCK, LF_ITEMS><<<ntiles, LF_BLOCK, 0, s>>>(L, n, cbase, lf, status, counter, epoch, ntiles);
+void launch_fill_blk(uint16_t *blk, const BlockInfo *blocks, uint32_t B, uint32_t N, hipStream_t s) {
+    (void)N;
+    if (B) k_fill_blk<<<dim3(64, B), 256, 0, s>>>(blk, blocks);
 }
-void launch_walk_count(const uint32_t *lf, uint32_t n, uint32_t P, uint32_t dshift, uint32_t nreg, uint32_t S,
+void launch_lf_pass(const uint8_t *L, Tiling tl, uint32_t ntiles, const uint32_t *cbase, uint32_t *lf,
+                    uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s) {
+    if (ntiles)
+        k_lf_pass<LF_BLOCK, LF_ITEMS><<<ntiles, LF_BLOCK, 0, s>>>(L, tl, cbase, lf, status, counter, epoch, ntiles);
+}
+void launch_walk_count(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
                        const uint32_t *cbase, uint64_t *link, uint32_t *len, uint8_t *stage, uint32_t cap,
                        uint32_t *cont, hipStream_t s) {
-    k_walk_count<<<(S + 1 + 255) / 256, 256, 0, s>>>(lf, n, P, dshift, nreg, S, cbase, link, len, stage, cap, cont);
+    if (ntiles) k_walk_count<<<ntiles, WALK_TILE, 0, s>>>(lf, tl, dshift, S, cbase, link, len, stage, cap, cont);
 }
 void launch_rank_jump(const uint64_t *in, uint64_t *out, uint32_t S, hipStream_t s) {
     k_rank_jump<<<(S + 1 + 255) / 256, 256, 0, s>>>(in, out, S);
 }
-void launch_walk_emit(const uint32_t *lf, uint32_t n, uint32_t P, uint32_t S, const uint64_t *link,
-                      const uint32_t *len, const uint32_t *cbase, const uint8_t *stage, uint32_t cap,
-                      const uint32_t *cont, uint8_t *out, hipStream_t s) {
-    k_walk_emit<<<(S + 255) / 256, 256, 0, s>>>(lf, n, P, S, link, len, cbase, stage, cap, cont, out);
+void launch_check_lists(const uint64_t *link, const BlockInfo *blocks, uint32_t B, uint32_t S, uint32_t *bad,
+                        hipStream_t s) {
+    if (B) k_check_lists<<<(B + 255) / 256, 256, 0, s>>>(link, blocks, B, S, bad);
+}
+void launch_walk_emit(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
+                      const uint64_t *link, const uint32_t *len, const uint32_t *cbase, const uint8_t *stage,
+                      uint32_t cap, const uint32_t *cont, const uint32_t *bad, uint8_t *out, hipStream_t s) {
+    if (ntiles)
+        k_walk_emit<<<ntiles, WALK_TILE, 0, s>>>(lf, tl, dshift, S, link, len, cbase, stage, cap, cont, bad, out);
+}
+void launch_zero_bad(uint8_t *out, const uint16_t *blk, const uint32_t *bad, uint32_t N, hipStream_t s) {
+    if (N) k_zero_bad<<<(N + 255) / 256, 256, 0, s>>>(out, blk, bad, N);
 }
 void launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s) {
     size_t n16 = bytes / 16;

@@ -1,0 +1,351 @@
+// Doubling rounds over groups (gfx950).
+//
+// After the first sort, the suffixes that still share their first h symbols form groups: contiguous suffix-array
+// ranges [head, head + size) with size >= 2, listed as (head, size) descriptors. A round with offset h gives every
+// suffix i of a group the key sec(i) = ISA[i + h] + h (or end - 1 - i when i + h lies past its block's end, below every
+// rank, shorter suffix first), sorts each group by sec, and splits it where sec changes: the subgroup heads become
+// the new ranks (ISA), the rows are rewritten in SA with their BWT bytes, and subgroups of two or more form the next
+// round's list. All keys of a round are gathered before any rank of the round changes.
+//
+// Small groups (at most SEG_TILE suffixes) never leave one workgroup: a tile owns the groups whose first element
+// offset (prefix sum of sizes over the list) lies in its SEG_TILE-wide range, gathers them into LDS and sorts them
+// there by (group, sec) with an LDS radix sort, so a round costs a few bytes per suffix instead of the global radix
+// passes. Larger groups take the global path (k_round_keys, radix passes, k_regroup) and come back here through
+// k_elems_to_groups.
+#include "kbwt_device.h"
+#include "kbwt_internal.h"
+
+namespace kbwt {
+
+// First index g in [0, G) with go[g] >= x (G when none).
+__device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *go, uint32_t G, uint32_t x) {
+    uint32_t lo = 0, hi = G;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (go[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Element rows of compacted active rows (pos_out, grp_out: SA position and group head of each, in SA order, groups
+// contiguous) to group descriptors: the last row of each group emits (head, size = pos - head + 1) into the small
+// list (size <= SEG_TILE) or the large list. Appends use one atomic per wave; the lists' order is free. *mp: rows.
+__global__ __launch_bounds__(256) void k_elems_to_groups(const uint32_t *__restrict__ pos, const uint32_t *__restrict__ grp,
+                                                         const uint32_t *__restrict__ mp, uint32_t *__restrict__ sh,
+                                                         uint32_t *__restrict__ ss, uint32_t *__restrict__ lh,
+                                                         uint32_t *__restrict__ ls, uint32_t *__restrict__ cnt) {
+    const uint32_t m = *mp;
+    const size_t stride = (size_t)gridDim.x * 256;
+    const size_t start = (size_t)blockIdx.x * 256 + threadIdx.x;
+    // every lane of a wave runs the same number of iterations (ballots below)
+    const size_t iters = (m + stride - 1) / stride;
+    for (size_t it = 0; it < iters; it++) {
+        const size_t o = start + it * stride;
+        bool last = false;
+        uint32_t head = 0, size = 0;
+        if (o < m) {
+            head = grp[o];
+            last = (o + 1 == m) || grp[o + 1] != head;
+            size = pos[o] - head + 1;
+        }
+        const bool small = last && size <= (uint32_t)SEG_TILE, large = last && size > (uint32_t)SEG_TILE;
+        const uint64_t bs = __ballot(small), bl = __ballot(large);
+        const uint32_t lane = __lane_id();
+        uint32_t base_s = 0, base_l = 0;
+        if (lane == 0) {
+            if (bs) base_s = atomicAdd(&cnt[0], (uint32_t)__popcll(bs));
+            if (bl) base_l = atomicAdd(&cnt[1], (uint32_t)__popcll(bl));
+        }
+        base_s = __shfl(base_s, 0, WAVE);
+        base_l = __shfl(base_l, 0, WAVE);
+        if (small) {
+            uint32_t at = base_s + (uint32_t)__popcll(bs & lanemask_lt());
+            sh[at] = head;
+            ss[at] = size;
+        }
+        if (large) {
+            uint32_t at = base_l + (uint32_t)__popcll(bl & lanemask_lt());
+            lh[at] = head;
+            ls[at] = size;
+        }
+    }
+}
+
+// Exclusive prefix sum of the group sizes: go[g] = sizes before g; the last tile writes the total to *tot.
+__global__ __launch_bounds__(256) void k_scan_groups(const uint32_t *__restrict__ gs, uint32_t G,
+                                                     uint32_t *__restrict__ go, uint32_t *__restrict__ tot,
+                                                     uint64_t *__restrict__ status, uint32_t *__restrict__ counter,
+                                                     uint32_t epoch, uint32_t ntiles) {
+    constexpr int ITEMS = 8, TILE = 256 * ITEMS;
+    __shared__ uint32_t s_scratch[8];
+    __shared__ uint32_t s_tile, s_ex;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_tile = next_tile(counter, ntiles);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const size_t base = (size_t)tile * TILE + (size_t)tid * ITEMS;   // blocked: thread owns ITEMS consecutive
+    uint32_t v[ITEMS], sum = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        v[k] = base + k < G ? gs[base + k] : 0u;
+        sum += v[k];
+    }
+    uint32_t total;
+    uint32_t ex = block_excl_sum<256>(sum, s_scratch, &total);
+    if (tid < WAVE) {
+        uint32_t prev = 0;
+        if (tile == 0) {
+            if (tid == 0) st_store(status, st_pack(epoch, ST_INC, total));
+        } else {
+            if (tid == 0) st_store(status + tile, st_pack(epoch, ST_AGG, total));
+            prev = lookback_wave<false>(status, tile, epoch);
+            if (tid == 0) st_store(status + tile, st_pack(epoch, ST_INC, prev + total));
+        }
+        if (tid == 0) {
+            s_ex = prev;
+            if (tile == ntiles - 1) *tot = prev + total;
+        }
+    }
+    __syncthreads();
+    uint32_t run = s_ex + ex;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        if (base + k < G) go[base + k] = run;
+        run += v[k];
+    }
+}
+
+// Rows of the large groups as element lists for the global path: pos = head + k, grp = head, in list order.
+// lo[g] = first row of large group g in the element list (host prefix sums), G large groups, m rows.
+__global__ __launch_bounds__(256) void k_large_elems(const uint32_t *__restrict__ lh, const uint32_t *__restrict__ lo,
+                                                     uint32_t G, uint32_t m, uint32_t *__restrict__ pos,
+                                                     uint32_t *__restrict__ grp) {
+    for (size_t a = (size_t)blockIdx.x * 256 + threadIdx.x; a < m; a += (size_t)gridDim.x * 256) {
+        uint32_t g = lower_bound_u32(lo, G, (uint32_t)a + 1) - 1;   // last g with lo[g] <= a
+        pos[a] = lh[g] + ((uint32_t)a - lo[g]);
+        grp[a] = lh[g];
+    }
+}
+
+// Sort key of suffix i at offset h inside its block [.., end).
+__device__ __forceinline__ uint32_t sec_of(const uint32_t *ISA, uint32_t i, uint32_t h, uint32_t end) {
+    const uint64_t j = (uint64_t)i + h;
+    return j < end ? ISA[j] + h : end - 1u - i;
+}
+
+// One tile of small groups. MODE 0 gathers sec[p] for every row p of the tile's groups; MODE 1 sorts each group by
+// sec in LDS, rewrites SA, ISA and the BWT bytes of its rows, and appends the subgroups of two or more to the next
+// list (nh, ns; *ncnt counts them).
+template <int MODE>
+__global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ gh, const uint32_t *__restrict__ go,
+                                                   uint32_t G, uint32_t E, uint32_t *__restrict__ SA,
+                                                   uint32_t *__restrict__ ISA, uint32_t *__restrict__ sec,
+                                                   uint8_t *__restrict__ L, const uint8_t *__restrict__ T,
+                                                   const uint16_t *__restrict__ blk,
+                                                   const BlockInfo *__restrict__ blocks, uint32_t n, uint32_t h,
+                                                   uint32_t *__restrict__ nh, uint32_t *__restrict__ ns,
+                                                   uint32_t *__restrict__ ncnt) {
+    constexpr int CAP = 2 * SEG_TILE;          // a tile's rows: groups starting in SEG_TILE, each <= SEG_TILE rows
+    constexpr int MAXG = SEG_TILE / 2 + 1;     // groups starting in the tile (two rows or more each)
+    constexpr int ITEMS = CAP / 256;
+    constexpr int W = 4;
+    __shared__ uint32_t s_gh[MAXG], s_go[MAXG];
+    __shared__ uint32_t s_g0, s_g1, s_cnt;
+    __shared__ uint64_t s_buf[MODE ? 2 * CAP : 1];
+    __shared__ uint32_t s_i[MODE ? CAP : 1];
+    __shared__ uint32_t s_wcnt[W][256];
+    __shared__ uint32_t s_start[256];
+    __shared__ uint32_t s_scratch[16];
+    __shared__ uint64_t s_or[W], s_and[W];
+    __shared__ uint32_t s_base;
+
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+    const uint32_t lo = blockIdx.x * (uint32_t)SEG_TILE, hi = lo + (uint32_t)SEG_TILE;
+    if (tid == 0) {
+        s_g0 = lower_bound_u32(go, G, lo);
+        s_g1 = lower_bound_u32(go, G, hi);
+    }
+    __syncthreads();
+    const uint32_t g0 = s_g0, g1 = s_g1, ng = g1 - g0;
+    if (ng == 0) return;
+    const uint32_t span0 = go[g0], span1 = g1 < G ? go[g1] : E;
+    const uint32_t cnt = span1 - span0;        // <= CAP - 1
+    for (uint32_t j = tid; j < ng; j += 256) {
+        s_gh[j] = gh[g0 + j];
+        s_go[j] = go[g0 + j] - span0;
+    }
+    __syncthreads();
+
+    // gather: row e of the tile belongs to group lg (binary search over the tile's group offsets)
+    uint64_t ko = 0, ka = ~0ull;
+    for (uint32_t e = tid; e < cnt; e += 256) {
+        uint32_t a = 0, b = ng;                // last lg with s_go[lg] <= e
+        while (b - a > 1) {
+            uint32_t mid = (a + b) >> 1;
+            if (s_go[mid] <= e) a = mid; else b = mid;
+        }
+        const uint32_t p = s_gh[a] + (e - s_go[a]);
+        const uint32_t i = SA[p];
+        if (MODE == 0) {
+            uint32_t end = n;
+            if (blk) { const BlockInfo bi = blocks[blk[p]]; end = bi.off + bi.n; }
+            sec[p] = sec_of(ISA, i, h, end);
+        } else {
+            const uint64_t key = (((uint64_t)a << 32) | sec[p]) << 12 | e;   // group, sec, row (sorts as a unit)
+            s_buf[e] = key;
+            s_i[e] = i;
+            ko |= key;
+            ka &= key;
+        }
+    }
+    if (MODE == 0) return;
+
+    // which key bits vary over the tile (bits 12 and up: the row index below is carried, not sorted)
+    for (int o = 1; o < WAVE; o <<= 1) {
+        ko |= __shfl_xor(ko, o, WAVE);
+        ka &= __shfl_xor(ka, o, WAVE);
+    }
+    if (lane == 0) { s_or[wave] = ko; s_and[wave] = ka; }
+    __syncthreads();
+    const uint64_t vary = ((s_or[0] | s_or[1] | s_or[2] | s_or[3]) ^ (s_and[0] & s_and[1] & s_and[2] & s_and[3])) >> 12;
+
+    // LDS radix sort of the packed keys over the varying 8-bit digits (stable multisplit per pass)
+    int src = 0;
+    for (uint32_t dsh = 0; dsh < 52; dsh += 8) {
+        if (((vary >> dsh) & 0xFFull) == 0) continue;
+        const uint32_t shift = 12 + dsh;
+        for (int i = tid; i < W * 256; i += 256) (&s_wcnt[0][0])[i] = 0;
+        __syncthreads();
+        uint64_t v[ITEMS];
+        uint32_t rk[ITEMS];
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t slot = wave * WAVE * ITEMS + k * WAVE + lane;
+            const bool valid = slot < cnt;
+            v[k] = valid ? s_buf[src * CAP + slot] : 0ull;
+            const uint32_t d = (uint32_t)(v[k] >> shift) & 255u;
+            const uint64_t peers = match_digit8(d, __ballot(valid));
+            rk[k] = 0;
+            if (valid) {
+                const uint32_t before = (uint32_t)__popcll(peers & lanemask_lt());
+                const uint32_t c = s_wcnt[wave][d];
+                rk[k] = c + before;
+                s_wcnt[wave][d] = c + (uint32_t)__popcll(peers);
+            }
+        }
+        __syncthreads();
+        uint32_t tot = 0;
+#pragma unroll
+        for (int w2 = 0; w2 < W; w2++) { uint32_t c = s_wcnt[w2][tid]; s_wcnt[w2][tid] = tot; tot += c; }
+        const uint32_t st = block_excl_sum<256>(tot, s_scratch, nullptr);
+        s_start[tid] = st;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t slot = wave * WAVE * ITEMS + k * WAVE + lane;
+            if (slot < cnt) {
+                const uint32_t d = (uint32_t)(v[k] >> shift) & 255u;
+                s_buf[(src ^ 1) * CAP + s_start[d] + s_wcnt[wave][d] + rk[k]] = v[k];
+            }
+        }
+        __syncthreads();
+        src ^= 1;
+    }
+
+    // split the groups: blocked rows per thread; a subgroup starts where (group, sec) changes, its head is the SA
+    // position of its first slot (max-scan over slots: the groups' heads themselves come in any order), and each
+    // kept subgroup's last slot emits it into the next list
+    constexpr int PER = CAP / 256;
+    uint32_t pk[PER], hk[PER], ik[PER], gk[PER];
+    bool flag[PER], lastk[PER];
+    uint32_t run_max = 0, nemit = 0;
+    const uint64_t *buf = s_buf + src * CAP;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t slot = tid * PER + k;
+        flag[k] = false;
+        lastk[k] = false;
+        pk[k] = 0;
+        ik[k] = 0;
+        gk[k] = 0;
+        if (slot < cnt) {
+            const uint64_t key = buf[slot] >> 12;
+            const uint32_t lg = (uint32_t)(key >> 32);
+            gk[k] = lg;
+            pk[k] = s_gh[lg] + (slot - s_go[lg]);
+            ik[k] = s_i[buf[slot] & 0xFFFu];
+            flag[k] = slot == 0 || (buf[slot - 1] >> 12) != key;
+            lastk[k] = slot + 1 == cnt || (buf[slot + 1] >> 12) != key;
+        }
+        const uint32_t cand = flag[k] ? slot : 0u;
+        run_max = run_max > cand ? run_max : cand;
+        hk[k] = run_max;   // within-thread running max of subgroup-start slots, fixed below with earlier threads
+    }
+    uint32_t ex_s, in_m, tot_s, tot_m;
+    // the max over earlier threads: inclusive max of each thread's run_max, shifted by one thread
+    block_scan_sum_max<256>(0u, run_max, ex_s, in_m, tot_s, tot_m, s_scratch);
+    const uint32_t prev_max = __shfl_up(in_m, 1, WAVE);
+    __shared__ uint32_t s_wmax[W];
+    if (lane == WAVE - 1) s_wmax[wave] = in_m;
+    __syncthreads();
+    uint32_t before_me = lane > 0 ? prev_max : (wave > 0 ? s_wmax[wave - 1] : 0u);
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t slot = tid * PER + k;
+        if (slot < cnt) {
+            const uint32_t hslot = hk[k] > before_me ? hk[k] : before_me;
+            const uint32_t head = s_gh[gk[k]] + (hslot - s_go[gk[k]]);
+            hk[k] = head;
+            SA[pk[k]] = ik[k];
+            ISA[ik[k]] = head;
+            L[pk[k]] = ik[k] ? T[ik[k] - 1] : (uint8_t)0;
+            if (lastk[k] && !flag[k]) nemit++;
+        }
+    }
+    uint32_t total_emit;
+    const uint32_t my_emit = block_excl_sum<256>(nemit, s_scratch, &total_emit);
+    if (tid == 0) s_base = total_emit ? atomicAdd(ncnt, total_emit) : 0u;
+    __syncthreads();
+    uint32_t at = s_base + my_emit;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const uint32_t slot = tid * PER + k;
+        if (slot < cnt && lastk[k] && !flag[k]) {
+            nh[at] = hk[k];
+            ns[at] = pk[k] - hk[k] + 1;
+            at++;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launch wrappers
+
+void launch_elems_to_groups(const uint32_t *pos, const uint32_t *grp, const uint32_t *mp, uint32_t mmax,
+                            uint32_t *sh, uint32_t *ss, uint32_t *lh, uint32_t *ls, uint32_t *cnt, hipStream_t s) {
+    uint32_t grid = (mmax + 255) / 256;
+    if (grid > 4096) grid = 4096;
+    if (grid) k_elems_to_groups<<<grid, 256, 0, s>>>(pos, grp, mp, sh, ss, lh, ls, cnt);
+}
+void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *tot, uint64_t *status,
+                        uint32_t *counter, uint32_t epoch, hipStream_t s) {
+    const uint32_t ntiles = (G + 2047) / 2048;
+    if (ntiles) k_scan_groups<<<ntiles, 256, 0, s>>>(gs, G, go, tot, status, counter, epoch, ntiles);
+}
+void launch_large_elems(const uint32_t *lh, const uint32_t *lo, uint32_t G, uint32_t m, uint32_t *pos, uint32_t *grp,
+                        hipStream_t s) {
+    uint32_t grid = (m + 255) / 256;
+    if (grid > 4096) grid = 4096;
+    if (grid) k_large_elems<<<grid, 256, 0, s>>>(lh, lo, G, m, pos, grp);
+}
+void launch_seg_round(int mode, const SegRoundArgs &a, hipStream_t s) {
+    const uint32_t ntiles = (a.E + SEG_TILE - 1) / SEG_TILE;
+    if (!ntiles) return;
+    if (mode == 0)
+        k_seg_round<0><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.G, a.E, a.SA, a.ISA, a.sec, a.L, a.T, a.blk, a.blocks, a.n,
+                                              a.h, a.nh, a.ns, a.ncnt);
+    else
+        k_seg_round<1><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.G, a.E, a.SA, a.ISA, a.sec, a.L, a.T, a.blk, a.blocks, a.n,
+                                              a.h, a.nh, a.ns, a.ncnt);
+}
+
+}  // namespace kbwt
