@@ -45,6 +45,11 @@ KERNEL_BYTES = {
     "walk_emit": 2.0,      # one staged byte read and written per step
     "rank_jump": 24.0,     # read two links, write one
     "fill_blk": 2.0,
+    "to_groups": 10.0,     # read pos 4 + group head 4 per active row, write (head, size) 8 per group (<= 4 per row)
+    "scan_groups": 8.0,    # read a group size, write its first-row offset
+    "large_elems": 8.0,    # write pos + group head per row of a large group
+    "seg_gather": 12.0,    # read the row's suffix 4 and ISA[i+h] 4, write the sort key 4
+    "seg_sort": 18.0,      # read key 4 + suffix 4, write SA 4, ISA 4, row byte 1, read T[i-1] 1
 }
 
 WORKLOADS = {

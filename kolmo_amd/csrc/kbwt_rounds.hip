@@ -29,45 +29,56 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *go, uint32_t
 
 // Element rows of compacted active rows (pos_out, grp_out: SA position and group head of each, in SA order, groups
 // contiguous) to group descriptors: the last row of each group emits (head, size = pos - head + 1) into the small
-// list (size <= SEG_TILE) or the large list. Appends use one atomic per wave; the lists' order is free. *mp: rows.
+// list (size <= SEG_TILE) or the large list. A workgroup takes tiles of E2G_TILE rows and makes one atomic append per
+// list and tile (appends to one counter from every wave serialise at the memory side); the lists' order is free.
+// *mp: rows.
+constexpr int E2G_ITEMS = 16, E2G_TILE = 256 * E2G_ITEMS;
 __global__ __launch_bounds__(256) void k_elems_to_groups(const uint32_t *__restrict__ pos, const uint32_t *__restrict__ grp,
                                                          const uint32_t *__restrict__ mp, uint32_t *__restrict__ sh,
                                                          uint32_t *__restrict__ ss, uint32_t *__restrict__ lh,
                                                          uint32_t *__restrict__ ls, uint32_t *__restrict__ cnt) {
+    __shared__ uint32_t s_scratch[8];
+    __shared__ uint32_t s_base[2];
     const uint32_t m = *mp;
-    const size_t stride = (size_t)gridDim.x * 256;
-    const size_t start = (size_t)blockIdx.x * 256 + threadIdx.x;
-    // every lane of a wave runs the same number of iterations (ballots below)
-    const size_t iters = (m + stride - 1) / stride;
-    for (size_t it = 0; it < iters; it++) {
-        const size_t o = start + it * stride;
-        bool last = false;
-        uint32_t head = 0, size = 0;
-        if (o < m) {
-            head = grp[o];
-            last = (o + 1 == m) || grp[o + 1] != head;
-            size = pos[o] - head + 1;
+    const int tid = threadIdx.x;
+    for (size_t t0 = (size_t)blockIdx.x * E2G_TILE; t0 < m; t0 += (size_t)gridDim.x * E2G_TILE) {
+        const size_t b = t0 + (size_t)tid * E2G_ITEMS;   // blocked: a thread owns E2G_ITEMS consecutive rows
+        uint32_t head[E2G_ITEMS], size[E2G_ITEMS];
+        uint32_t kind = 0;                               // bit k: row k ends a group; bit 16 + k: that group is large
+        uint32_t ns = 0, nl = 0;
+#pragma unroll
+        for (int k = 0; k < E2G_ITEMS; k++) {
+            const size_t o = b + k;
+            head[k] = o < m ? grp[o] : 0u;
+            size[k] = o < m ? pos[o] - head[k] + 1 : 0u;
         }
-        const bool small = last && size <= (uint32_t)SEG_TILE, large = last && size > (uint32_t)SEG_TILE;
-        const uint64_t bs = __ballot(small), bl = __ballot(large);
-        const uint32_t lane = __lane_id();
-        uint32_t base_s = 0, base_l = 0;
-        if (lane == 0) {
-            if (bs) base_s = atomicAdd(&cnt[0], (uint32_t)__popcll(bs));
-            if (bl) base_l = atomicAdd(&cnt[1], (uint32_t)__popcll(bl));
+        const uint32_t next = b + E2G_ITEMS < m ? grp[b + E2G_ITEMS] : 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < E2G_ITEMS; k++) {
+            const size_t o = b + k;
+            const uint32_t nh = k + 1 < E2G_ITEMS ? head[k + 1] : next;
+            const bool last = o < m && (o + 1 == m || nh != head[k]);
+            if (last) {
+                kind |= 1u << k;
+                if (size[k] > (uint32_t)SEG_TILE) { kind |= 1u << (16 + k); nl++; } else ns++;
+            }
         }
-        base_s = __shfl(base_s, 0, WAVE);
-        base_l = __shfl(base_l, 0, WAVE);
-        if (small) {
-            uint32_t at = base_s + (uint32_t)__popcll(bs & lanemask_lt());
-            sh[at] = head;
-            ss[at] = size;
+        uint32_t tot;
+        const uint32_t ex = block_excl_sum<256>(ns | (nl << 16), s_scratch, &tot);
+        if (tid == 0) {
+            s_base[0] = (tot & 0xFFFFu) ? atomicAdd(&cnt[0], tot & 0xFFFFu) : 0u;
+            s_base[1] = (tot >> 16) ? atomicAdd(&cnt[1], tot >> 16) : 0u;
         }
-        if (large) {
-            uint32_t at = base_l + (uint32_t)__popcll(bl & lanemask_lt());
-            lh[at] = head;
-            ls[at] = size;
+        __syncthreads();
+        uint32_t at_s = s_base[0] + (ex & 0xFFFFu), at_l = s_base[1] + (ex >> 16);
+#pragma unroll
+        for (int k = 0; k < E2G_ITEMS; k++) {
+            if (kind & (1u << k)) {
+                if (kind & (1u << (16 + k))) { lh[at_l] = head[k]; ls[at_l] = size[k]; at_l++; }
+                else { sh[at_s] = head[k]; ss[at_s] = size[k]; at_s++; }
+            }
         }
+        __syncthreads();   // s_base is rewritten by the next tile
     }
 }
 
@@ -322,8 +333,8 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
 
 void launch_elems_to_groups(const uint32_t *pos, const uint32_t *grp, const uint32_t *mp, uint32_t mmax,
                             uint32_t *sh, uint32_t *ss, uint32_t *lh, uint32_t *ls, uint32_t *cnt, hipStream_t s) {
-    uint32_t grid = (mmax + 255) / 256;
-    if (grid > 4096) grid = 4096;
+    uint32_t grid = (uint32_t)(((size_t)mmax + E2G_TILE - 1) / E2G_TILE);
+    if (grid > 2048) grid = 2048;
     if (grid) k_elems_to_groups<<<grid, 256, 0, s>>>(pos, grp, mp, sh, ss, lh, ls, cnt);
 }
 void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *tot, uint64_t *status,
