@@ -50,6 +50,8 @@ KERNEL_BYTES = {
     "large_elems": 8.0,    # write pos + group head per row of a large group
     "seg_gather": 12.0,    # read the row's suffix 4 and ISA[i+h] 4, write the sort key 4
     "seg_sort": 18.0,      # read key 4 + suffix 4, write SA 4, ISA 4, row byte 1, read T[i-1] 1
+    "isa_part": 16.0,      # read and write one (suffix, head) pair
+    "isa_scatter": 12.0,   # read a pair, write ISA
 }
 
 WORKLOADS = {
@@ -142,6 +144,18 @@ def roofline_from(kernels, steps, ms_step):
     return roofline, path, dom, achieved
 
 
+def model_bytes(n, st):
+    """Algorithmic bytes of one forward + inverse from the doubling profile (m_r = active[r] suffixes sorted in round r
+    with p_r radix passes): B_fwd = 13n + sum_r (8 + 24 p_r + 16) m_r + sum_{r>=1} 20 m_r + 6n, B_inv = 16n. Rounds
+    after the first sort groups of up to 1024 rows inside one workgroup, counted as one pass."""
+    act, passes = st["active"], st["passes"]
+    b = 13 * n + 6 * n + 16 * n
+    for r, m in enumerate(act):
+        p = passes[r] if r == 0 else max(1, passes[r])
+        b += (8 + 24 * p + 16) * m + (20 * m if r >= 1 else 0)
+    return b
+
+
 def timed_steps(K, steps, step):
     """Run `steps` steps with per-kernel profiling on; returns (seconds, {kernel: {ms, launches, elems}})."""
     import torch
@@ -201,7 +215,6 @@ def run_single(args, K):
                   and [int(x) for x in I] == entry["I"])
         if not parity:
             raise SystemExit("BWT differs from kolmo's reference output")
-    st = K.last_stats()
     if args.calib_copy:
         calib_copy(K, args.calib_copy, dev, stream)
 
@@ -214,7 +227,12 @@ def run_single(args, K):
     ms_step = elapsed / args.steps * 1e3
     value = n * args.steps / elapsed / 2**20
     roofline, path, dom, achieved = roofline_from(kernels, args.steps, ms_step)
-    log(f"bench: {value:.1f} MiB/s, {ms_step:.3f} ms/step, dominant {dom} {achieved:.0f} GB/s")
+    st = K.last_stats()
+    mb = model_bytes(n, st)
+    path["model_bytes_per_step"] = int(mb)
+    path["model_frac"] = round(mb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    log(f"bench: {value:.1f} MiB/s, {ms_step:.3f} ms/step, dominant {dom} {achieved:.0f} GB/s, "
+        f"path {path['model_frac']:.3f} of HBM peak")
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (kolmo's own C on this host)")
@@ -287,6 +305,9 @@ def run_batch(args, K):
     per_block_s = time.perf_counter() - t0
     per_block = total / per_block_s / 2**20
     roofline, path, dom, achieved = roofline_from(kernels, args.steps, ms_step)
+    mb = model_bytes(total, st)
+    path["model_bytes_per_step"] = int(mb)
+    path["model_frac"] = round(mb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     log(f"bench: batched {value:.1f} MiB/s, one call per block {per_block:.1f} MiB/s ({value / per_block:.2f}x)")
     cpu = None
     if not args.no_cpu:

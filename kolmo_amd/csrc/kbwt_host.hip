@@ -17,6 +17,8 @@
 namespace kbwt {
 namespace {
 
+constexpr size_t STAGE_MAX = (size_t)8 << 30;   // inverse: bytes of the staged sublist heads at most
+
 thread_local std::string g_last_error;
 thread_local kbwt_stats g_stats;
 
@@ -49,7 +51,7 @@ struct Workspace {
     std::mutex mu;
     uint32_t epoch = 1;
     Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, st_max, samples, stage, blk, counts, dbase,
-        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, lo;
+        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, lo, tstart;
     uint32_t *counters = nullptr;           // 16 tile counters (zeroed once)
     uint32_t *gcnt = nullptr;               // 4 group-list counters
     uint32_t *h_gcnt = nullptr;
@@ -355,6 +357,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         g.blk = blk;
         g.SA = nullptr;
         g.ISA = ISA;
+        g.pairs = K[P0 & 1];       // the other key buffer
         g.L = L;
         g.T = nullptr;
         g.pos_out = P<uint32_t>(w.pos[0]);
@@ -365,6 +368,15 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         g.counter = w.counters + 1;
         g.epoch = w.epoch++;
         TIMED("regroup", N, launch_regroup(g, s));
+        LAUNCHCHK();
+        // ISA[suffix] = head: partition the pairs by suffix window (one keys-only radix pass), then write window by
+        // window instead of scattering 4-byte writes over all of ISA
+        const uint32_t wshift = isa_window_shift(N);
+        TIMED("isa_part", N,
+              launch_partition_pairs(K[P0 & 1], K[(P0 - 1) & 1], N, wshift, P<uint64_t>(w.status), w.counters + 4,
+                                     w.epoch++, s));
+        LAUNCHCHK();
+        TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, wshift, ISA, s));
     }
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
@@ -384,9 +396,11 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         grow(w.ls[i], GMAX * 4);
     }
     grow(w.go, GMAX * 4);
+    grow(w.tstart, (NN / SEG_TILE + 2) * 4);
     grow(w.sec, NN * 4);
     grow(w.lo, GMAX * 8);
     uint32_t *gcnt = w.gcnt;          // [small groups, large groups, small rows] of the list being built
+    const uint32_t rank_max = env_u32("KBWT_SEG_RANK", SEG_RANK_MAX, 0, SEG_TILE);
     std::vector<uint32_t> h_lh, h_ls, h_lo;
     auto read_lists = [&](int c, uint32_t &Gs, uint32_t &Gl) {
         HIPCHK(hipMemcpyAsync(w.h_gcnt, gcnt, 16, hipMemcpyDeviceToHost, s));
@@ -421,8 +435,8 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         // gather every key of the round before any rank changes
         if (Gs) {
             TIMED("scan_groups", Gs,
-                  launch_scan_groups(P<uint32_t>(w.gs[cur]), Gs, P<uint32_t>(w.go), w.m_out, P<uint64_t>(w.st_sum),
-                                     w.counters + 3, w.epoch++, s));
+                  launch_scan_groups(P<uint32_t>(w.gs[cur]), Gs, P<uint32_t>(w.go), w.m_out, P<uint32_t>(w.tstart),
+                                     P<uint64_t>(w.st_sum), w.counters + 3, w.epoch++, s));
             LAUNCHCHK();
             HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
@@ -431,6 +445,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         SegRoundArgs sa{};
         sa.gh = P<uint32_t>(w.gh[cur]);
         sa.go = P<uint32_t>(w.go);
+        sa.tstart = P<uint32_t>(w.tstart);
         sa.G = Gs;
         sa.E = Es;
         sa.SA = SA;
@@ -445,6 +460,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         sa.nh = P<uint32_t>(w.gh[cur ^ 1]);
         sa.ns = P<uint32_t>(w.gs[cur ^ 1]);
         sa.ncnt = gcnt;
+        sa.rank_max = rank_max;
         if (Gs) TIMED("seg_gather", Es, launch_seg_round(0, sa, s));
         Shifts sh{};
         if (Gl) {
@@ -563,8 +579,9 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     st.n = (int32_t)N;
     st.blocks = (int32_t)B;
     // one splitter row in D = 2^dshift; the first cap bytes of each sublist are staged during the counting walk
+    // (the walk's time is its longest sublist, about D * ln(splitters) steps, times the load latency)
     const uint32_t dshift = env_u32("KBWT_INV_DSHIFT", 6, 2, 12);
-    const uint32_t cap = env_u32("KBWT_INV_CAP", 4u << dshift, 1, 1u << 16);
+    uint32_t cap = env_u32("KBWT_INV_CAP", 16u << dshift, 1, 1u << 16);
     const uint32_t dmask = (1u << dshift) - 1u;
     uint32_t S = 0;
     for (uint32_t b = 0; b < B; b++) {
@@ -580,7 +597,10 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     grow(w.pos[1], ((size_t)S + 1) * 4);       // continuation rows
     grow(w.K0, ((size_t)S + 1) * 8);           // links (ping)
     grow(w.K1, ((size_t)S + 1) * 8);           // links (pong)
-    grow(w.stage, (size_t)S * cap);
+    // step-major stage: cap rows of sstride bytes, at most STAGE_MAX bytes in all
+    const size_t sstride = ((size_t)S + 63) & ~(size_t)63;
+    if ((size_t)cap * sstride > STAGE_MAX) cap = (uint32_t)std::max<size_t>(1, STAGE_MAX / sstride);
+    grow(w.stage, (size_t)cap * sstride);
     const size_t ntiles_max = (NN + LF_TILE - 1) / LF_TILE + B;
     grow(w.status, ntiles_max * 256 * 8, true);
     grow(w.counts, (size_t)B * MAXP * 256 * 4, true);
@@ -623,9 +643,10 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     uint32_t *len = P<uint32_t>(w.pos[0]), *cont = P<uint32_t>(w.pos[1]);
     uint8_t *stage = P<uint8_t>(w.stage);
     TIMED("walk_count", N,
-          launch_walk_count(lf, twalk, nwt, dshift, S, P<uint32_t>(w.cbase), link[0], len, stage, cap, cont, s));
+          launch_walk_count(lf, twalk, nwt, dshift, S, P<uint32_t>(w.cbase), link[0], len, stage, sstride, cap, cont,
+                            s));
     LAUNCHCHK();
-    const uint32_t iters = ceil_log2(S + 1) + 1;
+    const uint32_t iters = rank_jump_iters(S);
     int cur = 0;
     for (uint32_t it = 0; it < iters; it++) {
         TIMED("rank_jump", S, launch_rank_jump(link[cur], link[cur ^ 1], S, s));
@@ -633,8 +654,8 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     }
     launch_check_lists(link[cur], d_bl, B, S, P<uint32_t>(w.bad), s);
     TIMED("walk_emit", N,
-          launch_walk_emit(lf, twalk, nwt, dshift, S, link[cur], len, P<uint32_t>(w.cbase), stage, cap, cont,
-                           P<uint32_t>(w.bad), dOut, s));
+          launch_walk_emit(lf, twalk, nwt, dshift, S, link[cur], len, P<uint32_t>(w.cbase), stage, sstride, cap,
+                           cont, P<uint32_t>(w.bad), dOut, s));
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(h_bad, w.bad.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -1026,7 +1047,7 @@ void kbwt_release(void) {
                    &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.st_max, &w.samples,
                    &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
                    &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
-                   &w.go,      &w.sec,    &w.lo};
+                   &w.go,      &w.sec,    &w.lo,     &w.tstart};
     for (Buf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;

@@ -17,6 +17,7 @@ constexpr int HIST_GRID = 512;               // workgroups of the histogram kern
 constexpr int WALK_TILE = 256;               // splitters per workgroup of the inverse walks
 constexpr uint32_t MIN_TILE = 2048;          // smallest tile of any tiled kernel (sizes the lookback arrays)
 constexpr int SEG_TILE = 1024;               // doubling rounds: groups of up to SEG_TILE rows sort inside one workgroup
+constexpr int SEG_RANK_MAX = 16;             // default: tiles whose groups all have at most this many rows rank directly
 
 // One block of a batch as the kernels see it.
 struct BlockInfo {
@@ -73,6 +74,7 @@ struct RegroupArgs {
     uint32_t m, kshift;
     const uint16_t *blk;          // block of every packed index, or nullptr for one block
     uint32_t *SA, *ISA;
+    uint64_t *pairs;              // when set: (suffix << 32 | head) per row instead of the ISA writes
     uint8_t *L;
     const uint8_t *T;
     uint32_t *pos_out, *grp_out, *m_out;
@@ -82,6 +84,7 @@ struct RegroupArgs {
 
 struct SegRoundArgs {
     const uint32_t *gh, *go;      // the small groups: heads and first-row offsets (prefix sums of sizes)
+    const uint32_t *tstart;       // first group of every SEG_TILE-row tile, and G at the end
     uint32_t G, E;                // groups, rows
     uint32_t *SA, *ISA, *sec;
     uint8_t *L;
@@ -90,12 +93,13 @@ struct SegRoundArgs {
     const BlockInfo *blocks;
     uint32_t n, h;
     uint32_t *nh, *ns, *ncnt;     // next round's small list and its counter
+    uint32_t rank_max;            // tiles whose groups have at most rank_max rows rank them directly
 };
 
 void launch_elems_to_groups(const uint32_t *pos, const uint32_t *grp, const uint32_t *mp, uint32_t mmax,
                             uint32_t *sh, uint32_t *ss, uint32_t *lh, uint32_t *ls, uint32_t *cnt, hipStream_t s);
-void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *tot, uint64_t *status,
-                        uint32_t *counter, uint32_t epoch, hipStream_t s);
+void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *tot, uint32_t *tstart,
+                        uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s);
 void launch_large_elems(const uint32_t *lh, const uint32_t *lo, uint32_t G, uint32_t m, uint32_t *pos, uint32_t *grp,
                         hipStream_t s);
 void launch_seg_round(int mode, const SegRoundArgs &a, hipStream_t s);
@@ -110,6 +114,10 @@ void launch_hist_keys(const uint64_t *K, uint32_t m, const Shifts &sh, uint32_t 
 uint32_t sort_tile(int cfg);
 void launch_sort_pass(const SortArgs &a, uint32_t ntiles, hipStream_t s);
 void launch_regroup(const RegroupArgs &a, hipStream_t s);
+uint32_t isa_window_shift(uint32_t N);
+void launch_partition_pairs(const uint64_t *in, uint64_t *out, uint32_t N, uint32_t wshift, uint64_t *status,
+                            uint32_t *counter, uint32_t epoch, hipStream_t s);
+void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t wshift, uint32_t *ISA, hipStream_t s);
 void launch_round_keys(const uint32_t *pos, const uint32_t *grp, uint32_t m, const uint32_t *SA, const uint32_t *ISA,
                        uint32_t n, const uint16_t *blk, const BlockInfo *blocks, uint32_t h, uint64_t *K, uint32_t *V,
                        unsigned long long *orand, hipStream_t s);
@@ -121,14 +129,16 @@ void launch_fill_blk(uint16_t *blk, const BlockInfo *blocks, uint32_t B, uint32_
 void launch_lf_pass(const uint8_t *L, Tiling tl, uint32_t ntiles, const uint32_t *cbase, uint32_t *lf,
                     uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s);
 void launch_walk_count(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
-                       const uint32_t *cbase, uint64_t *link, uint32_t *len, uint8_t *stage, uint32_t cap,
-                       uint32_t *cont, hipStream_t s);
+                       const uint32_t *cbase, uint64_t *link, uint32_t *len, uint8_t *stage, size_t sstride,
+                       uint32_t cap, uint32_t *cont, hipStream_t s);
+uint32_t rank_jump_iters(uint32_t S);
 void launch_rank_jump(const uint64_t *in, uint64_t *out, uint32_t S, hipStream_t s);
 void launch_check_lists(const uint64_t *link, const BlockInfo *blocks, uint32_t B, uint32_t S, uint32_t *bad,
                         hipStream_t s);
 void launch_walk_emit(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
                       const uint64_t *link, const uint32_t *len, const uint32_t *cbase, const uint8_t *stage,
-                      uint32_t cap, const uint32_t *cont, const uint32_t *bad, uint8_t *out, hipStream_t s);
+                      size_t sstride, uint32_t cap, const uint32_t *cont, const uint32_t *bad, uint8_t *out,
+                      hipStream_t s);
 void launch_zero_bad(uint8_t *out, const uint16_t *blk, const uint32_t *bad, uint32_t N, hipStream_t s);
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 void launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s);

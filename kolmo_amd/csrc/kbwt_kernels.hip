@@ -74,33 +74,44 @@ __device__ __forceinline__ TileRange tile_range(const Tiling &tl, uint32_t t, ui
 
 __global__ __launch_bounds__(256) void k_byte_hist(const uint8_t *__restrict__ T, Tiling tl, uint32_t ntiles,
                                                    uint32_t *__restrict__ counts) {
-    __shared__ uint32_t h[4][256];
-    const int tid = threadIdx.x, wave = tid >> 6;
-    for (int i = tid; i < 1024; i += 256) (&h[0][0])[i] = 0;
+    // BH_COPIES sub-histograms, lane l counting into copy l % BH_COPIES: equal bytes in one wave collide only within a
+    // copy (text is skewed; one histogram would serialise its frequent bytes)
+    constexpr int BH_COPIES = 16;
+    __shared__ uint32_t h[BH_COPIES][256];
+    const int tid = threadIdx.x, cp = tid % BH_COPIES;
+    for (int i = tid; i < BH_COPIES * 256; i += 256) (&h[0][0])[i] = 0;
     __syncthreads();
     const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
     const uint32_t t0 = blockIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
     int cur = -1;
+    auto flush = [&] {
+        __syncthreads();
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < BH_COPIES; k++) { c += h[k][tid]; h[k][tid] = 0; }
+        if (c) atomicAdd(&counts[(size_t)cur * 256 + tid], c);
+        __syncthreads();
+    };
     for (uint32_t t = t0; t < t1; t++) {
         TileRange r = tile_range(tl, t, HIST_TILE, 0);
         if ((int)r.b != cur) {
-            if (cur >= 0) {
-                __syncthreads();
-                uint32_t c = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
-                if (c) atomicAdd(&counts[(size_t)cur * 256 + tid], c);
-                h[0][tid] = h[1][tid] = h[2][tid] = h[3][tid] = 0;
-                __syncthreads();
-            }
+            if (cur >= 0) flush();
             cur = (int)r.b;
         }
-        for (size_t e = r.beg + tid; e < r.end; e += 256) atomicAdd(&h[wave][T[e]], 1u);
+        if (r.end - r.beg == (size_t)HIST_TILE && (r.beg & 15) == 0) {   // 16 bytes per thread
+            const uint4 v = reinterpret_cast<const uint4 *>(T + r.beg)[tid];
+            const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+#pragma unroll
+                for (int k = 0; k < 4; k++) atomicAdd(&h[cp][(wd[q] >> (8 * k)) & 255u], 1u);
+        } else {
+            for (size_t e = r.beg + tid; e < r.end; e += 256) atomicAdd(&h[cp][T[e]], 1u);
+        }
     }
-    __syncthreads();
-    if (cur >= 0) {
-        uint32_t c = h[0][tid] + h[1][tid] + h[2][tid] + h[3][tid];
-        if (c) atomicAdd(&counts[(size_t)cur * 256 + tid], c);
-    }
+    if (cur >= 0) flush();
 }
+
 
 // One workgroup per histogram set (set = block * sets_per_block + p): tot = counts, excl = base + exclusive scan of
 // counts, and counts is cleared. base: 0 (mode 0), the block's first packed index (1), its first row + 1 (2).
@@ -132,11 +143,12 @@ __device__ __forceinline__ uint64_t build_key(const uint8_t *c, uint8_t bwt_byte
 __global__ __launch_bounds__(256) void k_hist_T(const uint8_t *__restrict__ T, Tiling tl, uint32_t ntiles,
                                                 const uint8_t *__restrict__ codes, uint32_t P,
                                                 uint32_t *__restrict__ counts) {
-    __shared__ uint32_t h[MAXP][256];
+    constexpr int HT_COPIES = 4;   // sub-histogram sets, lane l counting into set l % HT_COPIES (see k_byte_hist)
+    __shared__ uint32_t h[HT_COPIES][MAXP][256];
     __shared__ uint8_t s_code[256];
     __shared__ uint8_t s_c[HIST_TILE + 64];
-    const int tid = threadIdx.x;
-    for (int i = tid; i < MAXP * 256; i += 256) (&h[0][0])[i] = 0;
+    const int tid = threadIdx.x, cp = tid % HT_COPIES;
+    for (int i = tid; i < HT_COPIES * MAXP * 256; i += 256) (&h[0][0][0])[i] = 0;
     const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
     const uint32_t t0 = blockIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
     int cur = -1;
@@ -147,9 +159,10 @@ __global__ __launch_bounds__(256) void k_hist_T(const uint8_t *__restrict__ T, T
         if ((int)r.b != cur) {
             if (cur >= 0) {
                 for (uint32_t i = tid; i < P * 256; i += 256) {
-                    uint32_t c = (&h[0][0])[i];
+                    uint32_t c = 0;
+#pragma unroll
+                    for (int k = 0; k < HT_COPIES; k++) { c += (&h[k][0][0])[i]; (&h[k][0][0])[i] = 0; }
                     if (c) atomicAdd(&counts[(size_t)cur * P * 256 + i], c);
-                    (&h[0][0])[i] = 0;
                 }
             }
             cur = (int)r.b;
@@ -166,13 +179,15 @@ __global__ __launch_bounds__(256) void k_hist_T(const uint8_t *__restrict__ T, T
         for (uint32_t j = tid; j < HIST_TILE; j += 256) {
             if (r.beg + j >= r.end) break;
             uint64_t key = build_key(s_c + j, 0, h0, w);
-            for (uint32_t p = 0; p < P; p++) atomicAdd(&h[p][(uint32_t)(key >> (8 * p + 8)) & 255u], 1u);
+            for (uint32_t p = 0; p < P; p++) atomicAdd(&h[cp][p][(uint32_t)(key >> (8 * p + 8)) & 255u], 1u);
         }
     }
     __syncthreads();
     if (cur >= 0)
         for (uint32_t i = tid; i < P * 256; i += 256) {
-            uint32_t c = (&h[0][0])[i];
+            uint32_t c = 0;
+#pragma unroll
+            for (int k = 0; k < HT_COPIES; k++) c += (&h[k][0][0])[i];
             if (c) atomicAdd(&counts[(size_t)cur * P * 256 + i], c);
         }
 }
@@ -200,7 +215,9 @@ __global__ __launch_bounds__(256) void k_hist_keys(const uint64_t *__restrict__ 
 // Stable pass over (key, value) pairs on the 8-bit digit at `shift`, segmented by block when tl.first is set.
 // FROM_T: keys are built from T with the block's code table (value = the suffix index). dbase[b*dstride + d]: start of
 // digit d in block b. status: one lookback word per (tile, digit); counter: tile-id counter.
-template <int BLOCK, int ITEMS, int LB, bool FROM_T>
+// Keys-only passes (VALS == false) move the 64-bit keys alone. dbase == nullptr gives every digit d the base
+// min(d << dstride, m): the bases of a partition of a permutation of [0, m) by its bits from dstride up.
+template <int BLOCK, int ITEMS, int LB, bool FROM_T, bool VALS = true>
 __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                      uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                      uint32_t m, uint32_t shift, const uint32_t *__restrict__ dbase,
@@ -228,7 +245,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     const TileRange tr = tile_range(tl, tile, TILE, m);
     const size_t base = tr.beg;
     const uint32_t cnt = (uint32_t)(tr.end - tr.beg);
-    const uint32_t *db = dbase + (size_t)tr.b * dstride;
+    const uint32_t *db = dbase ? dbase + (size_t)tr.b * dstride : nullptr;
 
     uint64_t key[ITEMS];
     uint32_t val[ITEMS];
@@ -259,7 +276,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
-            if (local < cnt) { key[k] = kin[base + local]; val[k] = vin[base + local]; }
+            if (local < cnt) { key[k] = kin[base + local]; val[k] = VALS ? vin[base + local] : 0u; }
             else { key[k] = 0; val[k] = 0; }
         }
     }
@@ -297,7 +314,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
             st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, ST_INC, excl + tot));
         }
         s_tstart[tid] = tstart;
-        s_gbase[tid] = db[tid] + excl - tstart;
+        const uint32_t dbv = db ? db[tid] : (uint32_t)min((uint64_t)tid << dstride, (uint64_t)m);
+        s_gbase[tid] = dbv + excl - tstart;
     }
     __syncthreads();
 
@@ -325,6 +343,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
             kout[s_gbase[dg[k]] + j] = kk;
         }
     }
+    if (!VALS) return;
     __syncthreads();
     uint32_t *s_v = reinterpret_cast<uint32_t *>(s_buf);
 #pragma unroll
@@ -344,7 +363,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
 
 // Sorted keys K[0..m) with values V (suffixes). Row a goes to SA position pos[a] (pos == nullptr: a). Rows a-1 and a
 // share a group when their keys agree above bit kshift (and, with blk, lie in the same block). Writes SA[pos[a]] =
-// V[a] (when SA != nullptr), ISA[V[a]] = head of its group (SA position of the group's first row), the BWT byte of
+// V[a] (when SA != nullptr), ISA[V[a]] = head of its group (SA position of the group's first row; with pairs, the
+// pair (V[a] << 32 | head) goes to pairs[a] instead, for k_isa_scatter after a partition by suffix), the BWT byte of
 // row pos[a] into L (the key's low byte when T == nullptr, else T[V[a]-1]; the byte of a block's first suffix is
 // never used), and compacts the rows of groups of two or more into (pos_out, grp_out); the last tile writes their
 // count to *m_out.
@@ -352,7 +372,8 @@ template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ K, const uint32_t *__restrict__ V,
                                                    const uint32_t *__restrict__ pos, uint32_t m, uint32_t kshift,
                                                    const uint16_t *__restrict__ blk, uint32_t *__restrict__ SA,
-                                                   uint32_t *__restrict__ ISA, uint8_t *__restrict__ L,
+                                                   uint32_t *__restrict__ ISA, uint64_t *__restrict__ pairs,
+                                                   uint8_t *__restrict__ L,
                                                    const uint8_t *__restrict__ T, uint32_t *__restrict__ pos_out,
                                                    uint32_t *__restrict__ grp_out, uint32_t *__restrict__ m_out,
                                                    uint64_t *__restrict__ st_sum, uint64_t *__restrict__ st_max,
@@ -436,7 +457,8 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
             uint32_t head = ex_max > lhead[k] ? ex_max : lhead[k];
             uint32_t i = V[a];
             if (SA) SA[lp[k]] = i;
-            ISA[i] = head;
+            if (pairs) pairs[a] = ((uint64_t)i << 32) | head;
+            else ISA[i] = head;
             L[lp[k]] = T ? (i ? T[i - 1] : (uint8_t)0) : lbyte[k];
             if (lkept[k]) {
                 uint32_t o = ex_sum + lkeep[k];
@@ -445,6 +467,32 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
             }
         }
     }
+}
+
+// ------------------------------------------------------------------ ISA from partitioned pairs
+
+// pairs: (suffix << 32 | head) partitioned by suffix >> wshift, so window w (suffixes [w << wshift, (w+1) << wshift))
+// holds exactly the pairs at the same indexes (the suffixes are a permutation of [0, N)). ISA[suffix] = head. The
+// writes of one window land in 4 * 2^wshift bytes of ISA; the windows are dealt to the workgroups so that window w is
+// written by the XCD w % 8 only (workgroups go to the XCDs round-robin), whose L2 merges them into whole lines.
+constexpr int SC_ITEMS = 16, SC_TILE = 256 * SC_ITEMS, NXCD = 8;
+__global__ __launch_bounds__(256) void k_isa_scatter(const uint64_t *__restrict__ pairs, uint32_t N, uint32_t wshift,
+                                                     uint32_t chunks, uint32_t NW, uint32_t *__restrict__ ISA) {
+    const uint32_t x = blockIdx.x % NXCD, k = blockIdx.x / NXCD;
+    const uint32_t win = (k / chunks) * NXCD + x, chunk = k % chunks;
+    if (win >= NW) return;
+    const size_t w0 = (size_t)win << wshift;
+    const size_t wend = min(w0 + ((size_t)1 << wshift), (size_t)N);
+    const size_t b0 = w0 + (size_t)chunk * SC_TILE + threadIdx.x;
+    uint64_t v[SC_ITEMS];
+#pragma unroll
+    for (int u = 0; u < SC_ITEMS; u++) {
+        const size_t a = b0 + (size_t)u * 256;
+        v[u] = a < wend ? pairs[a] : ~0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < SC_ITEMS; u++)
+        if (v[u] != ~0ull) ISA[(uint32_t)(v[u] >> 32)] = (uint32_t)v[u];
 }
 
 // ------------------------------------------------------------------ doubling round keys
@@ -602,13 +650,17 @@ __device__ __forceinline__ uint8_t row_byte(const uint32_t *s_c, uint32_t v) {
 
 // Splitters of block b: local j < nreg is the row j*D, j == nreg the row P (only when P % D != 0); global id
 // spoff + j. From each, walk LF to the next splitter row of the block; link = (steps << 32) | next splitter, len =
-// steps. The bytes met on the way (the text backwards from the splitter's suffix) go to stage[id*cap ..] for the
-// first cap steps; cont keeps the local row of step cap of a longer sublist. The marker row P ends the list:
-// link = (0, S). Workgroups take WALK_TILE splitters of one block.
+// steps. The bytes met on the way (the text backwards from the splitter's suffix) are staged step-major,
+// stage[t * sstride + id] for the first cap steps t, so that the lanes of a wave, which walk consecutive splitters
+// in step, store consecutive bytes; cont keeps the local row of step cap of a longer sublist. The marker row P ends
+// the list: link = (0, S). Workgroups take WALK_TILE splitters of one block.
+//
+// The walk is a chain of dependent loads, so its time is the longest sublist times the load latency: the load of
+// the next row is issued before the current row's byte is looked up and stored.
 __global__ __launch_bounds__(WALK_TILE) void k_walk_count(const uint32_t *__restrict__ lf, Tiling tl, uint32_t dshift,
                                                           uint32_t S, const uint32_t *__restrict__ cbase,
                                                           uint64_t *__restrict__ link, uint32_t *__restrict__ len,
-                                                          uint8_t *__restrict__ stage, uint32_t cap,
+                                                          uint8_t *__restrict__ stage, size_t sstride, uint32_t cap,
                                                           uint32_t *__restrict__ cont) {
     __shared__ uint32_t s_c[256];
     const uint32_t b = tile_block(tl.first, tl.B, blockIdx.x);
@@ -623,35 +675,48 @@ __global__ __launch_bounds__(WALK_TILE) void k_walk_count(const uint32_t *__rest
     if (j >= nsp) return;
     const uint32_t id = bi.spoff + j;
     const uint32_t sidP = bi.spoff + ((bi.P & dmask) == 0 ? (bi.P >> dshift) : bi.nreg);
-    uint32_t r = j < bi.nreg ? (j << dshift) : bi.P;
-    if (r == bi.P) { link[id] = S; len[id] = 0; return; }
-    uint32_t steps = 0;
-    uint8_t *st = stage + (size_t)id * cap;
-    do {
-        if (steps == cap) cont[id] = r;
-        uint32_t q = bi.off + (r < bi.P ? r : r - 1u);
-        uint32_t v = lf[q];
-        if (steps < cap) st[steps] = row_byte(s_c, v);
+    const uint32_t r0 = j < bi.nreg ? (j << dshift) : bi.P;
+    if (r0 == bi.P) { link[id] = S; len[id] = 0; return; }
+    uint32_t steps = 0, r = r0;
+    uint32_t v = lf[bi.off + r0 - (r0 < bi.P ? 0u : 1u)];   // LF of the row of step 0
+    uint8_t *st = stage + id;
+    for (;;) {
+        const uint32_t rn = v - bi.roff;                       // the row of step steps + 1
+        const bool more = (rn & dmask) != 0 && rn != bi.P && steps + 1 <= bi.n;
+        uint32_t vn = 0;
+        if (more) {
+            if (steps + 1 == cap) cont[id] = rn;
+            vn = lf[bi.off + rn - (rn < bi.P ? 0u : 1u)];
+        }
+        if (steps < cap) st[(size_t)steps * sstride] = row_byte(s_c, v);
         steps++;
-        r = v - bi.roff;
-    } while ((r & dmask) != 0 && r != bi.P && steps <= bi.n);
-    uint32_t nxt = r == bi.P ? sidP : bi.spoff + (r >> dshift);
+        r = rn;
+        if (!more) break;
+        v = vn;
+    }
+    const uint32_t nxt = r == bi.P ? sidP : bi.spoff + (r >> dshift);
     link[id] = ((uint64_t)steps << 32) | nxt;
     len[id] = steps;
 }
 
-// One synchronous pointer-jumping step: out[s] = (acc[s] + acc[ptr], ptr[ptr]) unless ptr == S (the end).
+// m-hop pointer jumping: out[s] = the node HOPS links on from s with the lengths on the way summed (the end S stops
+// a walk). Links that each cover d list nodes cover HOPS * d after one step.
+constexpr int RANK_HOPS = 4;
 __global__ __launch_bounds__(256) void k_rank_jump(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
                                                    uint32_t S) {
     size_t s = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (s > S) return;
     uint64_t v = in[s];
     uint32_t ptr = (uint32_t)v;
-    if (ptr != S) {
-        uint64_t w = in[ptr];
-        v = (((v >> 32) + (w >> 32)) << 32) | (uint32_t)w;
+    uint64_t acc = v >> 32;
+#pragma unroll
+    for (int hop = 1; hop < RANK_HOPS; hop++) {
+        if (ptr == S) break;
+        const uint64_t w = in[ptr];
+        acc += w >> 32;
+        ptr = (uint32_t)w;
     }
-    out[s] = v;
+    out[s] = (acc << 32) | ptr;
 }
 
 // A block's BWT is valid when the list from its row 0 (splitter spoff) reaches the end with n steps.
@@ -665,39 +730,73 @@ __global__ __launch_bounds__(256) void k_check_lists(const uint64_t *__restrict_
 }
 
 // Write every sublist: its splitter's suffix starts at block position pos = link >> 32 (after ranking), and step t of
-// the walk is the byte at pos-1-t. The first cap bytes come from stage; a longer sublist walks on from cont.
+// the walk is the byte at pos-1-t. The staged steps go through LDS, EMIT_ROWS steps of the workgroup's WALK_TILE
+// sublists at a time: rows of the step-major stage are read whole, and each wave then writes the bytes of one
+// sublist at a time as one contiguous run. A sublist longer than cap walks on from cont.
+constexpr int EMIT_ROWS = 64, EMIT_PITCH = WALK_TILE + 4;   // pitch: a wave's lanes read one column without conflicts
 __global__ __launch_bounds__(WALK_TILE) void k_walk_emit(const uint32_t *__restrict__ lf, Tiling tl, uint32_t dshift,
                                                          uint32_t S, const uint64_t *__restrict__ link,
                                                          const uint32_t *__restrict__ len,
                                                          const uint32_t *__restrict__ cbase,
-                                                         const uint8_t *__restrict__ stage, uint32_t cap,
-                                                         const uint32_t *__restrict__ cont,
+                                                         const uint8_t *__restrict__ stage, size_t sstride,
+                                                         uint32_t cap, const uint32_t *__restrict__ cont,
                                                          const uint32_t *__restrict__ bad, uint8_t *__restrict__ out) {
     __shared__ uint32_t s_c[256];
+    __shared__ uint32_t s_pos[WALK_TILE], s_st[WALK_TILE];
+    __shared__ uint8_t s_rows[EMIT_ROWS * EMIT_PITCH];
+    __shared__ uint32_t s_max;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
     const uint32_t b = tile_block(tl.first, tl.B, blockIdx.x);
     const uint32_t tb = blockIdx.x - tl.first[b];
     const BlockInfo bi = tl.blocks[b];
-    s_c[threadIdx.x] = cbase[(size_t)b * 256 + threadIdx.x];
-    __syncthreads();
-    if (bad[b]) return;
+    if (bad[b]) return;                                      // uniform over the workgroup
+    s_c[tid] = cbase[(size_t)b * 256 + tid];
+    if (tid == 0) s_max = 0;
     const uint32_t dmask = (1u << dshift) - 1u;
     const uint32_t nsp = bi.nreg + ((bi.P & dmask) != 0 ? 1u : 0u);
-    const uint32_t j = tb * WALK_TILE + threadIdx.x;
-    if (j >= nsp) return;
-    const uint32_t id = bi.spoff + j;
-    const uint32_t steps = len[id];
-    if (steps == 0) return;
-    const uint32_t pos = (uint32_t)(link[id] >> 32);
-    if (pos > bi.n || steps > pos) return;
-    const uint8_t *st = stage + (size_t)id * cap;
+    const uint32_t j0 = tb * WALK_TILE;
+    const uint32_t cnt = nsp - j0 < (uint32_t)WALK_TILE ? nsp - j0 : (uint32_t)WALK_TILE;
+    const uint32_t id0 = bi.spoff + j0;
+    uint32_t steps = 0, pos = 0;
+    if ((uint32_t)tid < cnt) {
+        steps = len[id0 + tid];
+        pos = (uint32_t)(link[id0 + tid] >> 32);
+        if (pos > bi.n || steps > pos) steps = 0;           // only a string that is no BWT gets here
+    }
     const uint32_t staged = steps < cap ? steps : cap;
-    uint8_t *o = out + bi.off + pos - 1;
-    for (uint32_t t = 0; t < staged; t++) o[-(int64_t)t] = st[t];
+    s_pos[tid] = pos;
+    s_st[tid] = staged;
+    __syncthreads();
+    if (staged) atomicMax(&s_max, staged);
+    __syncthreads();
+    const uint32_t rows = s_max;
+    for (uint32_t t0 = 0; t0 < rows; t0 += EMIT_ROWS) {
+        const uint32_t nr = rows - t0 < (uint32_t)EMIT_ROWS ? rows - t0 : (uint32_t)EMIT_ROWS;
+        if ((uint32_t)tid < cnt) {
+            const uint8_t *src = stage + (size_t)t0 * sstride + id0 + tid;
+            for (uint32_t t = 0; t < nr; t += 16) {      // 16 loads in flight per lane
+                uint8_t v[16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) v[u] = t + u < nr && t0 + t + u < staged ? src[(size_t)(t + u) * sstride] : 0;
+#pragma unroll
+                for (int u = 0; u < 16; u++)
+                    if (t + u < nr) s_rows[(t + u) * EMIT_PITCH + tid] = v[u];
+            }
+        }
+        __syncthreads();
+        for (int k = wave * WAVE; k < wave * WAVE + WAVE; k++) {
+            const uint32_t sk = s_st[k];
+            if (sk <= t0) continue;
+            const uint32_t t = t0 + (uint32_t)lane;
+            if (t < sk && (uint32_t)lane < nr) out[bi.off + s_pos[k] - 1 - t] = s_rows[lane * EMIT_PITCH + k];
+        }
+        __syncthreads();
+    }
     if (steps > cap) {
-        uint32_t r = cont[id];
+        uint8_t *o = out + bi.off + pos - 1;
+        uint32_t r = cont[id0 + tid];
         for (uint32_t t = cap; t < steps; t++) {
-            uint32_t q = bi.off + (r < bi.P ? r : r - 1u);
-            uint32_t v = lf[q];
+            const uint32_t v = lf[bi.off + r - (r < bi.P ? 0u : 1u)];
             o[-(int64_t)t] = row_byte(s_c, v);
             r = v - bi.roff;
         }
@@ -769,10 +868,32 @@ void launch_sort_pass(const SortArgs &a, uint32_t ntiles, hipStream_t s) {
     default: sort_pass_cfg<512, 16, 4>(a, ntiles, s); break;
     }
 }
+void launch_partition_pairs(const uint64_t *in, uint64_t *out, uint32_t N, uint32_t wshift, uint64_t *status,
+                            uint32_t *counter, uint32_t epoch, hipStream_t s) {
+    constexpr int BLOCK = 512, ITEMS = 16;
+    const uint32_t ntiles = (uint32_t)(((size_t)N + BLOCK * ITEMS - 1) / (BLOCK * ITEMS));
+    if (!ntiles) return;
+    k_sort_pass<BLOCK, ITEMS, 4, false, false><<<ntiles, BLOCK, 0, s>>>(
+        in, nullptr, out, nullptr, N, 32 + wshift, nullptr, wshift, Tiling{nullptr, nullptr, 1}, status, counter,
+        epoch, ntiles, nullptr, nullptr);
+}
+uint32_t isa_window_shift(uint32_t N) {
+    uint32_t b = 0;
+    while (((uint64_t)256 << b) < N) b++;   // at most 256 windows: one 8-bit digit
+    return b;
+}
+void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t wshift, uint32_t *ISA, hipStream_t s) {
+    if (!N) return;
+    const uint32_t NW = (uint32_t)(((uint64_t)N + ((uint64_t)1 << wshift) - 1) >> wshift);
+    const uint32_t chunks = (uint32_t)((((uint64_t)1 << wshift) + SC_TILE - 1) / SC_TILE);
+    const uint32_t grid = (NW + NXCD - 1) / NXCD * chunks * NXCD;
+    k_isa_scatter<<<grid, 256, 0, s>>>(pairs, N, wshift, chunks, NW, ISA);
+}
 void launch_regroup(const RegroupArgs &a, hipStream_t s) {
     const uint32_t ntiles = (a.m + RG_TILE - 1) / RG_TILE;
     if (!ntiles) return;
-    k_regroup<RG_BLOCK, RG_ITEMS><<<ntiles, RG_BLOCK, 0, s>>>(a.K, a.V, a.pos, a.m, a.kshift, a.blk, a.SA, a.ISA, a.L,
+    k_regroup<RG_BLOCK, RG_ITEMS><<<ntiles, RG_BLOCK, 0, s>>>(a.K, a.V, a.pos, a.m, a.kshift, a.blk, a.SA, a.ISA,
+                                                             a.pairs, a.L,
                                                              a.T, a.pos_out, a.grp_out, a.m_out, a.st_sum, a.st_max,
                                                              a.counter, a.epoch, ntiles);
 }
@@ -801,9 +922,17 @@ void launch_lf_pass(const uint8_t *L, Tiling tl, uint32_t ntiles, const uint32_t
         k_lf_pass<LF_BLOCK, LF_ITEMS><<<ntiles, LF_BLOCK, 0, s>>>(L, tl, cbase, lf, status, counter, epoch, ntiles);
 }
 void launch_walk_count(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
-                       const uint32_t *cbase, uint64_t *link, uint32_t *len, uint8_t *stage, uint32_t cap,
-                       uint32_t *cont, hipStream_t s) {
-    if (ntiles) k_walk_count<<<ntiles, WALK_TILE, 0, s>>>(lf, tl, dshift, S, cbase, link, len, stage, cap, cont);
+                       const uint32_t *cbase, uint64_t *link, uint32_t *len, uint8_t *stage, size_t sstride,
+                       uint32_t cap, uint32_t *cont, hipStream_t s) {
+    if (ntiles)
+        k_walk_count<<<ntiles, WALK_TILE, 0, s>>>(lf, tl, dshift, S, cbase, link, len, stage, sstride, cap, cont);
+}
+uint32_t rank_jump_iters(uint32_t S) {
+    // reach after k steps: RANK_HOPS^k list nodes; a list has at most S nodes
+    uint32_t k = 0;
+    uint64_t reach = 1;
+    while (reach < (uint64_t)S + 1) { reach *= RANK_HOPS; k++; }
+    return k + 1;
 }
 void launch_rank_jump(const uint64_t *in, uint64_t *out, uint32_t S, hipStream_t s) {
     k_rank_jump<<<(S + 1 + 255) / 256, 256, 0, s>>>(in, out, S);
@@ -814,9 +943,11 @@ void launch_check_lists(const uint64_t *link, const BlockInfo *blocks, uint32_t 
 }
 void launch_walk_emit(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
                       const uint64_t *link, const uint32_t *len, const uint32_t *cbase, const uint8_t *stage,
-                      uint32_t cap, const uint32_t *cont, const uint32_t *bad, uint8_t *out, hipStream_t s) {
+                      size_t sstride, uint32_t cap, const uint32_t *cont, const uint32_t *bad, uint8_t *out,
+                      hipStream_t s) {
     if (ntiles)
-        k_walk_emit<<<ntiles, WALK_TILE, 0, s>>>(lf, tl, dshift, S, link, len, cbase, stage, cap, cont, bad, out);
+        k_walk_emit<<<ntiles, WALK_TILE, 0, s>>>(lf, tl, dshift, S, link, len, cbase, stage, sstride, cap, cont, bad,
+                                                 out);
 }
 void launch_zero_bad(uint8_t *out, const uint16_t *blk, const uint32_t *bad, uint32_t N, hipStream_t s) {
     if (N) k_zero_bad<<<(N + 255) / 256, 256, 0, s>>>(out, blk, bad, N);

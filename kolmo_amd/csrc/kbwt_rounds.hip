@@ -82,9 +82,13 @@ __global__ __launch_bounds__(256) void k_elems_to_groups(const uint32_t *__restr
     }
 }
 
-// Exclusive prefix sum of the group sizes: go[g] = sizes before g; the last tile writes the total to *tot.
+// Exclusive prefix sum of the group sizes: go[g] = sizes before g; the last tile writes the total to *tot. Also the
+// first group of every SEG_TILE-row tile of k_seg_round, tstart[t] = the first g with go[g] >= t * SEG_TILE
+// (tstart[tiles] = G): group g, rows [go[g], go[g] + size) with size <= SEG_TILE, holds at most one tile start t *
+// SEG_TILE, whose first group is g when the tile starts with it and g + 1 otherwise.
 __global__ __launch_bounds__(256) void k_scan_groups(const uint32_t *__restrict__ gs, uint32_t G,
                                                      uint32_t *__restrict__ go, uint32_t *__restrict__ tot,
+                                                     uint32_t *__restrict__ tstart,
                                                      uint64_t *__restrict__ status, uint32_t *__restrict__ counter,
                                                      uint32_t epoch, uint32_t ntiles) {
     constexpr int ITEMS = 8, TILE = 256 * ITEMS;
@@ -121,7 +125,13 @@ __global__ __launch_bounds__(256) void k_scan_groups(const uint32_t *__restrict_
     uint32_t run = s_ex + ex;
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        if (base + k < G) go[base + k] = run;
+        const size_t g = base + k;
+        if (g < G) {
+            go[g] = run;
+            const uint32_t t = (run + (uint32_t)SEG_TILE - 1) / (uint32_t)SEG_TILE;   // first tile start >= run
+            if ((uint64_t)t * SEG_TILE < (uint64_t)run + v[k]) tstart[t] = (uint64_t)t * SEG_TILE == run ? (uint32_t)g : (uint32_t)g + 1;
+            if (g + 1 == G) tstart[(run + v[k] + (uint32_t)SEG_TILE - 1) / (uint32_t)SEG_TILE] = G;
+        }
         run += v[k];
     }
 }
@@ -149,19 +159,19 @@ __device__ __forceinline__ uint32_t sec_of(const uint32_t *ISA, uint32_t i, uint
 // list (nh, ns; *ncnt counts them).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ gh, const uint32_t *__restrict__ go,
-                                                   uint32_t G, uint32_t E, uint32_t *__restrict__ SA,
+                                                   const uint32_t *__restrict__ tstart, uint32_t G, uint32_t E, uint32_t *__restrict__ SA,
                                                    uint32_t *__restrict__ ISA, uint32_t *__restrict__ sec,
                                                    uint8_t *__restrict__ L, const uint8_t *__restrict__ T,
                                                    const uint16_t *__restrict__ blk,
                                                    const BlockInfo *__restrict__ blocks, uint32_t n, uint32_t h,
                                                    uint32_t *__restrict__ nh, uint32_t *__restrict__ ns,
-                                                   uint32_t *__restrict__ ncnt) {
+                                                   uint32_t *__restrict__ ncnt, uint32_t rank_max) {
     constexpr int CAP = 2 * SEG_TILE;          // a tile's rows: groups starting in SEG_TILE, each <= SEG_TILE rows
     constexpr int MAXG = SEG_TILE / 2 + 1;     // groups starting in the tile (two rows or more each)
     constexpr int ITEMS = CAP / 256;
     constexpr int W = 4;
     __shared__ uint32_t s_gh[MAXG], s_go[MAXG];
-    __shared__ uint32_t s_g0, s_g1, s_cnt;
+    __shared__ uint32_t s_g0, s_g1, s_cnt, s_maxsz;
     __shared__ uint64_t s_buf[MODE ? 2 * CAP : 1];
     __shared__ uint32_t s_i[MODE ? CAP : 1];
     __shared__ uint32_t s_wcnt[W][256];
@@ -171,10 +181,10 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     __shared__ uint32_t s_base;
 
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
-    const uint32_t lo = blockIdx.x * (uint32_t)SEG_TILE, hi = lo + (uint32_t)SEG_TILE;
     if (tid == 0) {
-        s_g0 = lower_bound_u32(go, G, lo);
-        s_g1 = lower_bound_u32(go, G, hi);
+        s_g0 = tstart[blockIdx.x];
+        s_g1 = tstart[blockIdx.x + 1];
+        s_maxsz = 0;
     }
     __syncthreads();
     const uint32_t g0 = s_g0, g1 = s_g1, ng = g1 - g0;
@@ -186,6 +196,8 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         s_go[j] = go[g0 + j] - span0;
     }
     __syncthreads();
+    if (MODE == 1)
+        for (uint32_t j = tid; j < ng; j += 256) atomicMax(&s_maxsz, (j + 1 < ng ? s_go[j + 1] : cnt) - s_go[j]);
 
     // gather: row e of the tile belongs to group lg (binary search over the tile's group offsets)
     uint64_t ko = 0, ka = ~0ull;
@@ -210,6 +222,53 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         }
     }
     if (MODE == 0) return;
+    __syncthreads();
+
+    // Tiles of small groups only: each row counts, within its group, the rows of smaller sec (less) and of equal sec
+    // before it (eqb). Its new SA position is head_of_group + less + eqb, its subgroup's head head_of_group + less,
+    // and the last row of a subgroup of two or more emits it. Same order as the radix path below (rows of equal sec
+    // keep their slot order). Both paths write nothing for a group that does not split, and the BWT byte of a row
+    // only when it ends up alone (its SA position is final then).
+    if (s_maxsz <= rank_max) {
+        constexpr int PERR = CAP / 256;
+        uint32_t eh[PERR], es[PERR];
+        uint32_t nemit = 0;
+#pragma unroll
+        for (int k = 0; k < PERR; k++) {
+            const uint32_t e = tid + k * 256;
+            es[k] = 0;
+            eh[k] = 0;
+            if (e < cnt) {
+                const uint64_t key = s_buf[e] >> 12;
+                const uint32_t lg = (uint32_t)(key >> 32), se = (uint32_t)key;
+                const uint32_t st = s_go[lg], en = lg + 1 < ng ? s_go[lg + 1] : cnt;
+                uint32_t less = 0, eq = 0, eqb = 0;
+                for (uint32_t j = st; j < en; j++) {
+                    const uint32_t sj = (uint32_t)(s_buf[j] >> 12);
+                    less += sj < se ? 1u : 0u;
+                    eq += sj == se ? 1u : 0u;
+                    eqb += (sj == se && j < e) ? 1u : 0u;
+                }
+                const uint32_t i = s_i[e];
+                const uint32_t head = s_gh[lg] + less, pnew = head + eqb;
+                if (eq != en - st) {             // the group splits (else SA, ISA stay as they are)
+                    SA[pnew] = i;
+                    ISA[i] = head;
+                    if (eq == 1) L[pnew] = i ? T[i - 1] : (uint8_t)0;   // a row's byte is written once it is alone
+                }
+                if (eq >= 2 && eqb == eq - 1) { eh[k] = head; es[k] = eq; nemit++; }
+            }
+        }
+        uint32_t total_emit;
+        const uint32_t my_emit = block_excl_sum<256>(nemit, s_scratch, &total_emit);
+        if (tid == 0) s_base = total_emit ? atomicAdd(ncnt, total_emit) : 0u;
+        __syncthreads();
+        uint32_t at = s_base + my_emit;
+#pragma unroll
+        for (int k = 0; k < PERR; k++)
+            if (es[k]) { nh[at] = eh[k]; ns[at] = es[k]; at++; }
+        return;
+    }
 
     // which key bits vary over the tile (bits 12 and up: the row index below is carried, not sorted)
     for (int o = 1; o < WAVE; o <<= 1) {
@@ -292,6 +351,12 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         run_max = run_max > cand ? run_max : cand;
         hk[k] = run_max;   // within-thread running max of subgroup-start slots, fixed below with earlier threads
     }
+    __shared__ uint32_t s_nsub[MAXG];
+    for (uint32_t j = tid; j < ng; j += 256) s_nsub[j] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PER; k++)
+        if (flag[k]) atomicAdd(&s_nsub[gk[k]], 1u);
     uint32_t ex_s, in_m, tot_s, tot_m;
     // the max over earlier threads: inclusive max of each thread's run_max, shifted by one thread
     block_scan_sum_max<256>(0u, run_max, ex_s, in_m, tot_s, tot_m, s_scratch);
@@ -307,9 +372,11 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
             const uint32_t hslot = hk[k] > before_me ? hk[k] : before_me;
             const uint32_t head = s_gh[gk[k]] + (hslot - s_go[gk[k]]);
             hk[k] = head;
-            SA[pk[k]] = ik[k];
-            ISA[ik[k]] = head;
-            L[pk[k]] = ik[k] ? T[ik[k] - 1] : (uint8_t)0;
+            if (s_nsub[gk[k]] > 1) {
+                SA[pk[k]] = ik[k];
+                ISA[ik[k]] = head;
+                if (flag[k] && lastk[k]) L[pk[k]] = ik[k] ? T[ik[k] - 1] : (uint8_t)0;
+            }
             if (lastk[k] && !flag[k]) nemit++;
         }
     }
@@ -337,10 +404,10 @@ void launch_elems_to_groups(const uint32_t *pos, const uint32_t *grp, const uint
     if (grid > 2048) grid = 2048;
     if (grid) k_elems_to_groups<<<grid, 256, 0, s>>>(pos, grp, mp, sh, ss, lh, ls, cnt);
 }
-void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *tot, uint64_t *status,
-                        uint32_t *counter, uint32_t epoch, hipStream_t s) {
+void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *tot, uint32_t *tstart,
+                        uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s) {
     const uint32_t ntiles = (G + 2047) / 2048;
-    if (ntiles) k_scan_groups<<<ntiles, 256, 0, s>>>(gs, G, go, tot, status, counter, epoch, ntiles);
+    if (ntiles) k_scan_groups<<<ntiles, 256, 0, s>>>(gs, G, go, tot, tstart, status, counter, epoch, ntiles);
 }
 void launch_large_elems(const uint32_t *lh, const uint32_t *lo, uint32_t G, uint32_t m, uint32_t *pos, uint32_t *grp,
                         hipStream_t s) {
@@ -352,11 +419,11 @@ void launch_seg_round(int mode, const SegRoundArgs &a, hipStream_t s) {
     const uint32_t ntiles = (a.E + SEG_TILE - 1) / SEG_TILE;
     if (!ntiles) return;
     if (mode == 0)
-        k_seg_round<0><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.G, a.E, a.SA, a.ISA, a.sec, a.L, a.T, a.blk, a.blocks, a.n,
-                                              a.h, a.nh, a.ns, a.ncnt);
+        k_seg_round<0><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.L, a.T, a.blk, a.blocks, a.n,
+                                              a.h, a.nh, a.ns, a.ncnt, a.rank_max);
     else
-        k_seg_round<1><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.G, a.E, a.SA, a.ISA, a.sec, a.L, a.T, a.blk, a.blocks, a.n,
-                                              a.h, a.nh, a.ns, a.ncnt);
+        k_seg_round<1><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.L, a.T, a.blk, a.blocks, a.n,
+                                              a.h, a.nh, a.ns, a.ncnt, a.rank_max);
 }
 
 }  // namespace kbwt

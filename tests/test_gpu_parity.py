@@ -155,3 +155,14 @@ def test_gpu_not_a_bwt(gpu):
     assert back.size == U.size
     back2 = K.unbwt_aux(U, U.size, np.array([1], np.int32))
     assert np.array_equal(back, back2)
+
+
+@pytest.mark.parametrize("rank_max", ["0", "2", "1024"])
+def test_gpu_round_paths(gpu, rank_max, monkeypatch):
+    """Both ways a doubling round splits small groups (radix sort in LDS, direct ranking), forced one way or the
+    other through KBWT_SEG_RANK, on inputs with many rounds of small and mid-sized groups."""
+    monkeypatch.setenv("KBWT_SEG_RANK", rank_max)
+    for kind, n in [("repeats", 1 << 20), ("dna", 700000), ("logs", 300000), ("csvcols", 200000), ("mixed", 500000)]:
+        T = K.generate(kind, 31, n)
+        say("round paths", rank_max, kind, n)
+        check_case(T, 1 << K.codec18_rlog(n), f"{kind} n {n} rank_max {rank_max}")
