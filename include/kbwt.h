@@ -73,6 +73,41 @@ KBWT_API int32_t kbwt_dev_bwt_aux_batch(const uint8_t *dT, uint8_t *dU, const kb
 KBWT_API int32_t kbwt_dev_unbwt_aux_batch(const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count,
                                           const int32_t *I, void *stream);
 
+/* ---- learned router (no counterpart in the reference) ----
+   Picks the coder of a level-8 block, the choice src/core/blocks.c:390-411 and :473-507 make today by coding a slice
+   of the block both ways: class 0 block sorting (codec 18, kolmo_bwt_compress), 1 KLZMA (codec 24,
+   kolmo_klzma_compress level 8), 2 store the block. The choice is recorded per block, so coders, containers and the
+   decoder are unchanged; exhaustive best-per-block (code both, keep the smaller) stays the fallback for blocks the
+   network is unsure of.
+   Features, KBWT_ROUTER_FEATURES fp32 per block: sqrt(byte count / n) for the 256 byte values, the order-1 entropy
+   H(T[i] | T[i-1]) / 8, and m_r / n for r = 1..5 (the share of suffixes still in rank groups at the start of
+   doubling round r). Network: 262 -> 64 -> 64 -> 3, ReLU, softmax; KBWT_ROUTER_PARAMS fp32 parameters, input-major
+   (W1[k*64+j], b1, W2[k*64+j], b2, W3[k*3+c], b3). All router arrays below are host memory. */
+#define KBWT_ROUTER_FEATURES 262
+#define KBWT_ROUTER_CLASSES 3
+#define KBWT_ROUTER_PARAMS 21187
+#define KBWT_ROUTER_GRAD_MAX 4096   /* samples of one kbwt_router_grad call and of one training batch */
+
+/* kbwt_dev_bwt_aux_batch, and the router's features of every block into features[count * KBWT_ROUTER_FEATURES]. */
+KBWT_API int32_t kbwt_dev_bwt_aux_batch_features(const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count,
+                                                 int32_t *I, float *features, void *stream);
+typedef struct {
+    int32_t epochs, batch;       /* passes over the samples; samples per Adam step (<= KBWT_ROUTER_GRAD_MAX) */
+    float lr, beta1, beta2, eps; /* Adam; 0 for beta1, beta2, eps takes 0.9, 0.999, 1e-8 */
+    uint64_t seed;               /* shuffling */
+} kbwt_router_train_cfg;
+/* Deterministic He-uniform initialisation from seed. */
+KBWT_API int32_t kbwt_router_init(float *params, uint64_t seed);
+/* Minibatch Adam on the weighted cross-entropy mean_s w[s] * -log p(y[s] | X[s]) (w NULL: all 1), y in 0..2.
+   params: in, trained out. epoch_loss (NULL or cfg->epochs floats): mean loss of each epoch. */
+KBWT_API int32_t kbwt_router_train(const float *X, const int32_t *y, const float *w, int32_t ns,
+                                   const kbwt_router_train_cfg *cfg, float *params, float *epoch_loss);
+/* Gradient of that mean loss at params over ns <= KBWT_ROUTER_GRAD_MAX samples, and the loss. */
+KBWT_API int32_t kbwt_router_grad(const float *X, const int32_t *y, const float *w, int32_t ns, const float *params,
+                                  float *grad, float *loss);
+/* Class probabilities, probs[ns * 3]. */
+KBWT_API int32_t kbwt_router_predict(const float *X, int32_t ns, const float *params, float *probs);
+
 /* ---- telemetry of the calling thread's last call ---- */
 #define KBWT_MAX_ROUNDS 40
 typedef struct {

@@ -2,10 +2,12 @@
 // forward suffix sort (round loop) and of the inverse, for a batch of blocks (one block is the batch of one).
 // Validation and return codes follow src/algo/sais/sais.c:12027-12082 of kolmo 0.0.2.
 #include "kbwt_internal.h"
+#include "kbwt_router.h"
 #include "../../include/kbwt.h"
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -51,7 +53,7 @@ struct Workspace {
     std::mutex mu;
     uint32_t epoch = 1;
     Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, st_max, samples, stage, blk, counts, dbase,
-        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, lo, tstart;
+        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, lo, tstart, act, rfeat, bigram, rmeta;
     uint32_t *counters = nullptr;           // 16 tile counters (zeroed once)
     uint32_t *gcnt = nullptr;               // 4 group-list counters
     uint32_t *h_gcnt = nullptr;
@@ -227,8 +229,42 @@ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 // bl: one BlockInfo per block with off, n (>= 2), r and soff filled in; packed input dT (N bytes), output dU.
 // I: host array of nsamp sampled indexes. Per-call metadata (blocks, tile tables, code tables) goes to the device
 // in one copy.
+// The rest of the router's features after the doubling rounds: m_r / n from the active counts, and the order-1
+// entropy from bigram histograms (blocks in groups of at most 1024, 256 KiB of histogram each).
+void router_features_tail(Workspace &w, const uint8_t *dT, const BlockInfo *d_bl, const std::vector<BlockInfo> &bl,
+                          const uint32_t *d_act, float *feat, hipStream_t s) {
+    const uint32_t B = (uint32_t)bl.size();
+    constexpr uint32_t CH = 1024;
+    grow(w.rfeat, (size_t)B * ROUTER_NF * 4);
+    grow(w.bigram, (size_t)std::min(B, CH) * 65536 * 4, true);   // every use clears what it filled
+    grow(w.rmeta, ((size_t)CH + 1) * 4);
+    std::vector<uint32_t> first(CH + 1);
+    for (uint32_t b0 = 0; b0 < B; b0 += CH) {
+        const uint32_t nb = std::min(CH, B - b0);
+        first[0] = 0;
+        for (uint32_t k = 0; k < nb; k++) first[k + 1] = first[k] + (bl[b0 + k].n + BIGRAM_TILE - 1) / BIGRAM_TILE;
+        HIPCHK(hipMemcpyAsync(w.rmeta.p, first.data(), ((size_t)nb + 1) * 4, hipMemcpyHostToDevice, s));
+        const Tiling tl{P<uint32_t>(w.rmeta), d_bl + b0, nb};
+        launch_order1(dT, tl, first[nb], d_bl + b0, nb, P<uint32_t>(w.bigram), P<float>(w.rfeat) + (size_t)b0 * ROUTER_NF,
+                      s);
+        LAUNCHCHK();
+        HIPCHK(hipStreamSynchronize(s));   // rmeta and first are reused by the next group
+    }
+    std::vector<float> ent((size_t)B * ROUTER_NF);
+    std::vector<uint32_t> act((size_t)ROUTER_ROUNDS * B);
+    HIPCHK(hipMemcpyAsync(ent.data(), w.rfeat.p, ent.size() * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(act.data(), d_act, act.size() * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (uint32_t b = 0; b < B; b++) {
+        float *f = feat + (size_t)b * ROUTER_NF;
+        f[256] = ent[(size_t)b * ROUTER_NF + 256];
+        for (int r = 0; r < ROUTER_ROUNDS; r++) f[257 + r] = (float)act[(size_t)r * B + b] / (float)bl[b].n;
+    }
+}
+
+// feat (host, B x ROUTER_NF, or nullptr): the router's features of every block, computed on the way.
 void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo> &bl, uint32_t N, int32_t *I,
-             uint32_t nsamp, hipStream_t s) {
+             uint32_t nsamp, hipStream_t s, float *feat = nullptr) {
     const uint32_t B = (uint32_t)bl.size();
     kbwt_stats &st = g_stats;
     memset(&st, 0, sizeof st);
@@ -310,6 +346,15 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     }
     memcpy(h_bl, bl.data(), B * sizeof(BlockInfo));
     HIPCHK(hipMemcpyAsync(dm, hm, bytes, hipMemcpyHostToDevice, s));
+    uint32_t *d_act = nullptr;
+    if (feat) {   // router features: the byte histogram now, the active shares per round, the order-1 entropy last
+        for (uint32_t b = 0; b < B; b++)
+            for (int c = 0; c < 256; c++)
+                feat[(size_t)b * ROUTER_NF + c] = std::sqrt((float)h_counts[(size_t)b * 256 + c] / (float)bl[b].n);
+        grow(w.act, (size_t)ROUTER_ROUNDS * B * 4);
+        d_act = P<uint32_t>(w.act);
+        launch_fill_u8(P<uint8_t>(w.act), 0, (size_t)ROUTER_ROUNDS * B * 4, s);
+    }
 
     // 2. digit histograms of the first keys
     TIMED("hist_T", N, launch_hist_T(dT, thist, nht, d_codes, P0, P<uint32_t>(w.counts), s));
@@ -385,9 +430,6 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
 
     // 5. doubling rounds over the groups of two or more (SA ranges). Small groups sort inside one workgroup each
     // (k_seg_round); groups of more than SEG_TILE rows take the global path (round keys, radix passes, regroup).
-    uint32_t hmin = 56;
-    for (uint32_t b = 0; b < B; b++) hmin = std::min(hmin, bl[b].h0);
-    uint64_t h = hmin;
     const size_t GMAX = NN / 2 + 2;   // groups have two rows or more
     for (int i = 0; i < 2; i++) {
         grow(w.gh[i], GMAX * 4);
@@ -430,7 +472,12 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     while (Gs + Gl > 0) {
         round++;
         if (round >= KBWT_MAX_ROUNDS) fail(hipErrorUnknown, "doubling did not converge");
-        const uint32_t hh = (uint32_t)std::min<uint64_t>(h, N);
+        if (d_act && round <= ROUTER_ROUNDS) {   // rows of each block still in groups at the start of this round
+            uint32_t *a = d_act + (size_t)(round - 1) * B;
+            launch_count_active(P<uint32_t>(w.gh[cur]), P<uint32_t>(w.gs[cur]), Gs, blk, a, s);
+            launch_count_active(P<uint32_t>(w.lh[cur]), P<uint32_t>(w.ls[cur]), Gl, blk, a, s);
+            LAUNCHCHK();
+        }
         uint32_t Es = 0, ml = 0;
         // gather every key of the round before any rank changes
         if (Gs) {
@@ -456,7 +503,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         sa.blk = blk;
         sa.blocks = d_bl;
         sa.n = N;
-        sa.h = hh;
+        sa.round = (uint32_t)round;
         sa.nh = P<uint32_t>(w.gh[cur ^ 1]);
         sa.ns = P<uint32_t>(w.gs[cur ^ 1]);
         sa.ncnt = gcnt;
@@ -480,7 +527,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
                                      P<uint32_t>(w.grp[0]), s));
             HIPCHK(hipMemcpyAsync(w.orand, w.h_orand_init, 16, hipMemcpyHostToDevice, s));
             TIMED("round_keys", ml,
-                  launch_round_keys(P<uint32_t>(w.pos[0]), P<uint32_t>(w.grp[0]), ml, SA, ISA, N, blk, d_bl, hh,
+                  launch_round_keys(P<uint32_t>(w.pos[0]), P<uint32_t>(w.grp[0]), ml, SA, ISA, N, blk, d_bl, (uint32_t)round,
                                     K[0], V[0], w.orand, s));
             LAUNCHCHK();
             HIPCHK(hipMemcpyAsync(w.h_orand, w.orand, 16, hipMemcpyDeviceToHost, s));
@@ -552,11 +599,11 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         st.active[round] = (int64_t)Es + ml;
         cur ^= 1;
         read_lists(cur, Gs, Gl);
-        h *= 2;
     }
     st.rounds = round;
 
     // 6. BWT bytes (the row bytes shifted around each block's primary row) and sampled indexes
+    if (feat) router_features_tail(w, dT, d_bl, bl, d_act, feat, s);
     TIMED("bwt", N, launch_bwt(dT, L, ISA, blk, d_bl, dU, N, s));
     TIMED("samples", nsamp, launch_samples(ISA, d_bl, B, nsamp, P<int32_t>(w.samples), s));
     LAUNCHCHK();
@@ -706,7 +753,7 @@ bool packed(const uint8_t *base, const kbwt_seg *segs, int32_t count) {
 
 // Forward on a batch whose bytes sit in device memory (dT, dU device pointers; I host).
 void forward_batch_dev(Workspace &w, const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count, int32_t *I,
-                       hipStream_t s) {
+                       hipStream_t s, float *feat = nullptr) {
     std::vector<uint32_t> soff(count + 1, 0);
     for (int32_t b = 0; b < count; b++) soff[b + 1] = soff[b] + nsamples(segs[b].n, segs[b].r);
     std::vector<BlockInfo> bl;
@@ -739,7 +786,11 @@ void forward_batch_dev(Workspace &w, const uint8_t *dT, uint8_t *dU, const kbwt_
             src = P<uint8_t>(w.T);
             dst = P<uint8_t>(w.U);
         }
-        forward(w, src, dst, bl, N, Id.data(), nsamp, s);
+        std::vector<float> fi(feat ? bl.size() * ROUTER_NF : 0);
+        forward(w, src, dst, bl, N, Id.data(), nsamp, s, feat ? fi.data() : nullptr);
+        if (feat)
+            for (size_t k = 0; k < bl.size(); k++)
+                memcpy(feat + (size_t)which[k] * ROUTER_NF, fi.data() + k * ROUTER_NF, ROUTER_NF * 4);
         if (!direct)
             for (size_t k = 0; k < bl.size(); k++)
                 HIPCHK(hipMemcpyAsync(dU + segs[which[k]].offset, P<uint8_t>(w.U) + bl[k].off, bl[k].n,
@@ -750,6 +801,16 @@ void forward_batch_dev(Workspace &w, const uint8_t *dT, uint8_t *dU, const kbwt_
             if (segs[b].n == 1)
                 HIPCHK(hipMemcpyAsync(dU + segs[b].offset, dT + segs[b].offset, 1, hipMemcpyDeviceToDevice, s));
             I[soff[b]] = segs[b].n;
+            if (feat) {   // nothing to sort: the histogram of the one byte, if any
+                float *f = feat + (size_t)b * ROUTER_NF;
+                memset(f, 0, ROUTER_NF * 4);
+                if (segs[b].n == 1) {
+                    uint8_t c = 0;
+                    HIPCHK(hipMemcpyAsync(&c, dT + segs[b].offset, 1, hipMemcpyDeviceToHost, s));
+                    HIPCHK(hipStreamSynchronize(s));
+                    f[c] = 1.f;
+                }
+            }
         }
     for (size_t k = 0; k < bl.size(); k++)
         memcpy(I + soff[which[k]], Id.data() + bl[k].soff, nsamples((int32_t)bl[k].n, (int32_t)bl[k].r) * 4);
@@ -813,6 +874,12 @@ bool samples_ok(const kbwt_seg *segs, int32_t count, const int32_t *I) {
 }
 
 }  // namespace
+
+void record_error(const char *msg) {
+    g_last_error = msg;
+    fprintf(stderr, "kbwt: %s\n", msg);
+}
+
 }  // namespace kbwt
 
 using namespace kbwt;
@@ -912,6 +979,16 @@ int32_t kbwt_dev_unbwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r,
     });
 }
 
+int32_t kbwt_dev_bwt_aux_batch_features(const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count,
+                                        int32_t *I, float *features, void *stream) {
+    if (dT == NULL || dU == NULL || I == NULL || features == NULL || !batch_ok(segs, count, true)) return -1;
+    return guarded([&] {
+        Workspace &w = workspace();
+        std::lock_guard<std::mutex> lk(w.mu);
+        hipStream_t s = stream ? (hipStream_t)stream : w.stream;
+        forward_batch_dev(w, dT, dU, segs, count, I, s, features);
+    });
+}
 int32_t kbwt_dev_bwt_aux_batch(const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count, int32_t *I,
                                void *stream) {
     if (dT == NULL || dU == NULL || I == NULL || !batch_ok(segs, count, true)) return -1;
@@ -1047,7 +1124,7 @@ void kbwt_release(void) {
                    &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.st_max, &w.samples,
                    &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
                    &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
-                   &w.go,      &w.sec,    &w.lo,     &w.tstart};
+                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta};
     for (Buf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;

@@ -40,6 +40,14 @@ struct Tiling {
     uint32_t B;
 };
 
+// Offset of doubling round r >= 1 in block bi: h0 * 2^(r-1) symbols, its sorted prefix after the round before (at
+// most n: past the block's end every suffix is told apart by its length). Each block doubles from its own h0, so a
+// block's rounds, and the share of its suffixes each round still sorts, do not depend on the rest of its batch.
+__host__ __device__ inline uint32_t round_offset(const BlockInfo &bi, uint32_t r) {
+    const uint64_t h = r - 1 < 40 ? (uint64_t)bi.h0 << (r - 1) : (uint64_t)bi.n;
+    return h < bi.n ? (uint32_t)h : bi.n;
+}
+
 struct Shifts {
     uint32_t P;
     uint32_t s[MAXP];
@@ -90,8 +98,8 @@ struct SegRoundArgs {
     uint8_t *L;
     const uint8_t *T;
     const uint16_t *blk;
-    const BlockInfo *blocks;
-    uint32_t n, h;
+    const BlockInfo *blocks;      // every block (blocks[0] when blk == nullptr)
+    uint32_t n, round;            // doubling round >= 1: offset round_offset(block, round)
     uint32_t *nh, *ns, *ncnt;     // next round's small list and its counter
     uint32_t rank_max;            // tiles whose groups have at most rank_max rows rank them directly
 };
@@ -119,7 +127,7 @@ void launch_partition_pairs(const uint64_t *in, uint64_t *out, uint32_t N, uint3
                             uint32_t *counter, uint32_t epoch, hipStream_t s);
 void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t wshift, uint32_t *ISA, hipStream_t s);
 void launch_round_keys(const uint32_t *pos, const uint32_t *grp, uint32_t m, const uint32_t *SA, const uint32_t *ISA,
-                       uint32_t n, const uint16_t *blk, const BlockInfo *blocks, uint32_t h, uint64_t *K, uint32_t *V,
+                       uint32_t n, const uint16_t *blk, const BlockInfo *blocks, uint32_t round, uint64_t *K, uint32_t *V,
                        unsigned long long *orand, hipStream_t s);
 void launch_bwt(const uint8_t *T, const uint8_t *L, const uint32_t *ISA, const uint16_t *blk, const BlockInfo *blocks,
                 uint8_t *U, uint32_t N, hipStream_t s);

@@ -503,7 +503,7 @@ __global__ __launch_bounds__(256) void k_round_keys(const uint32_t *__restrict__
                                                     uint32_t m, const uint32_t *__restrict__ SA,
                                                     const uint32_t *__restrict__ ISA, uint32_t n,
                                                     const uint16_t *__restrict__ blk,
-                                                    const BlockInfo *__restrict__ blocks, uint32_t h,
+                                                    const BlockInfo *__restrict__ blocks, uint32_t round,
                                                     uint64_t *__restrict__ K, uint32_t *__restrict__ V,
                                                     unsigned long long *__restrict__ orand) {
     __shared__ uint64_t s_o[4], s_a[4];
@@ -512,8 +512,8 @@ __global__ __launch_bounds__(256) void k_round_keys(const uint32_t *__restrict__
     for (size_t a = (size_t)blockIdx.x * 256 + threadIdx.x; a < m; a += stride) {
         uint32_t p = pos[a];
         uint32_t i = SA[p];
-        uint32_t end = n;
-        if (blk) { const BlockInfo bi = blocks[blk[p]]; end = bi.off + bi.n; }
+        const BlockInfo bi = blocks[blk ? blk[p] : 0u];
+        const uint32_t h = round_offset(bi, round), end = bi.off + bi.n;
         uint64_t j = (uint64_t)i + h;
         uint32_t sec = j < end ? ISA[j] + h : end - 1u - i;
         uint64_t key = ((uint64_t)grp[a] << 32) | sec;
@@ -898,11 +898,11 @@ void launch_regroup(const RegroupArgs &a, hipStream_t s) {
                                                              a.counter, a.epoch, ntiles);
 }
 void launch_round_keys(const uint32_t *pos, const uint32_t *grp, uint32_t m, const uint32_t *SA, const uint32_t *ISA,
-                       uint32_t n, const uint16_t *blk, const BlockInfo *blocks, uint32_t h, uint64_t *K, uint32_t *V,
+                       uint32_t n, const uint16_t *blk, const BlockInfo *blocks, uint32_t round, uint64_t *K, uint32_t *V,
                        unsigned long long *orand, hipStream_t s) {
     uint32_t grid = (m + 255) / 256;
     if (grid > 2048) grid = 2048;
-    if (grid) k_round_keys<<<grid, 256, 0, s>>>(pos, grp, m, SA, ISA, n, blk, blocks, h, K, V, orand);
+    if (grid) k_round_keys<<<grid, 256, 0, s>>>(pos, grp, m, SA, ISA, n, blk, blocks, round, K, V, orand);
 }
 void launch_bwt(const uint8_t *T, const uint8_t *L, const uint32_t *ISA, const uint16_t *blk, const BlockInfo *blocks,
                 uint8_t *U, uint32_t N, hipStream_t s) {

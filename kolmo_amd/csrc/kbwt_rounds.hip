@@ -163,7 +163,7 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
                                                    uint32_t *__restrict__ ISA, uint32_t *__restrict__ sec,
                                                    uint8_t *__restrict__ L, const uint8_t *__restrict__ T,
                                                    const uint16_t *__restrict__ blk,
-                                                   const BlockInfo *__restrict__ blocks, uint32_t n, uint32_t h,
+                                                   const BlockInfo *__restrict__ blocks, uint32_t n, uint32_t round,
                                                    uint32_t *__restrict__ nh, uint32_t *__restrict__ ns,
                                                    uint32_t *__restrict__ ncnt, uint32_t rank_max) {
     constexpr int CAP = 2 * SEG_TILE;          // a tile's rows: groups starting in SEG_TILE, each <= SEG_TILE rows
@@ -210,9 +210,8 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         const uint32_t p = s_gh[a] + (e - s_go[a]);
         const uint32_t i = SA[p];
         if (MODE == 0) {
-            uint32_t end = n;
-            if (blk) { const BlockInfo bi = blocks[blk[p]]; end = bi.off + bi.n; }
-            sec[p] = sec_of(ISA, i, h, end);
+            const BlockInfo bi = blocks[blk ? blk[p] : 0u];
+            sec[p] = sec_of(ISA, i, round_offset(bi, round), bi.off + bi.n);
         } else {
             const uint64_t key = (((uint64_t)a << 32) | sec[p]) << 12 | e;   // group, sec, row (sorts as a unit)
             s_buf[e] = key;
@@ -420,10 +419,10 @@ void launch_seg_round(int mode, const SegRoundArgs &a, hipStream_t s) {
     if (!ntiles) return;
     if (mode == 0)
         k_seg_round<0><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.L, a.T, a.blk, a.blocks, a.n,
-                                              a.h, a.nh, a.ns, a.ncnt, a.rank_max);
+                                              a.round, a.nh, a.ns, a.ncnt, a.rank_max);
     else
         k_seg_round<1><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.L, a.T, a.blk, a.blocks, a.n,
-                                              a.h, a.nh, a.ns, a.ncnt, a.rank_max);
+                                              a.round, a.nh, a.ns, a.ncnt, a.rank_max);
 }
 
 }  // namespace kbwt

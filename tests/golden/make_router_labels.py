@@ -1,0 +1,149 @@
+#!/usr/bin/env python3
+"""Labels of the learned router: for every block of the router workload (seed 5, 8192 blocks, sizes log-uniform
+256 KiB-4 MiB, six generators), the payload size and single-thread CPU time of kolmo's two level-8 coders, run from
+oracle/_ref (built from kolmo's own C sources):
+
+  - block sorting, codec 18: kolmo_bwt_compress with KOLMO_BWT_DEFAULTS (src/codecs/bwt_stream.c:211,
+    bwt_stream.h:37); a negative return (KB_NOT_COMPRESSIBLE) means the block is stored;
+  - KLZMA: kolmo_klzma_compress at native level 8 (src/algo/klzma.h:27; the level-9 role maps to native 8,
+    src/core/native.c:38); 0 means the block is stored.
+
+Exhaustive best-per-block takes the smaller of the two (or stores the block when neither is smaller than it).
+
+The method without the network is level 8's rule (src/core/blocks.c:390-402): a block that looks like text
+(looks_text_latin, src/plan/detect.c:32-42: >= 95% printable, tab, newline, CR or >= 0xC2 bytes in four 16 KB slices)
+goes to block sorting; any other block of 64 KiB or more codes a slice (the whole block up to 2 MiB, else 512 KiB from
+the middle, offset ((n - 512 KiB) / 2) & ~63) with block sorting and with KLZMA at the role level 1 (native level 1,
+src/codecs/native.c:38) and keeps block sorting only when it is at least 2% smaller. The second pass (--slices)
+records those slice sizes and times and the text flag.
+
+Output: router_labels.npz (no pickles): kind, seed, n, size_bwt, size_klzma (-1: stored), t_bwt, t_klzma (seconds);
+after --slices also text (0/1), slice_bwt, slice_klzma1 (-1: stored or not coded), t_slice_bwt, t_slice_klzma1.
+
+    python tests/golden/make_router_labels.py [--count 8192] [--procs 8]
+    python tests/golden/make_router_labels.py --slices [--procs 8]
+"""
+import argparse
+import ctypes
+import os
+import sys
+import time
+from multiprocessing import Pool
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+ROUTER_SEED, ROUTER_LO, ROUTER_HI = 5, 256 << 10, 4 << 20
+ROUTER_KINDS = ["markov2", "csvcols", "logs", "bf16", "random", "dna"]
+
+_lib = None
+
+
+def _init():
+    global _lib
+    import refs
+    _lib = refs.ref()
+    _lib.kolmo_klzma_compress.restype = ctypes.c_size_t
+    _lib.kolmo_klzma_compress.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                          ctypes.c_int]
+
+
+def _label(args):
+    kind, seed, n = args
+    import kolmo_amd as K
+    import refs
+    T = K.generate(int(kind), int(seed), int(n), model_seed=ROUTER_SEED)
+    t0 = time.perf_counter()
+    rc, _ = refs.bwt18_compress(_lib, T, threads=1)
+    t1 = time.perf_counter()
+    out = np.empty(T.size + (T.size >> 2) + 4096, np.uint8)
+    k = _lib.kolmo_klzma_compress(T.ctypes.data, T.size, out.ctypes.data, out.size, 8)
+    t2 = time.perf_counter()
+    return (rc if rc > 0 else -1), (int(k) if k > 0 else -1), t1 - t0, t2 - t1
+
+
+def looks_text_latin(T):
+    """src/plan/detect.c:32-42."""
+    n = T.size
+    if n < 4096:
+        return 0
+    tot = pr = 0
+    for k in range(4):
+        st = n // 4 * k
+        m = min(n - st, 16384)
+        c = T[st: st + m]
+        pr += int(np.count_nonzero(((c >= 32) & (c < 127)) | (c == 9) | (c == 10) | (c == 13) | (c >= 0xC2)))
+        tot += m
+    return int(pr * 100 >= tot * 95)
+
+
+def _slice(args):
+    kind, seed, n = args
+    import kolmo_amd as K
+    import refs
+    T = K.generate(int(kind), int(seed), int(n), model_seed=ROUTER_SEED)
+    text = looks_text_latin(T)
+    if text or n < (64 << 10):
+        return text, -1, -1, 0.0, 0.0
+    sn = n if n <= (2 << 20) else 512 << 10
+    so = ((n - sn) // 2) & ~63
+    S = np.ascontiguousarray(T[so: so + sn])
+    t0 = time.perf_counter()
+    rc, _ = refs.bwt18_compress(_lib, S, threads=1)
+    t1 = time.perf_counter()
+    out = np.empty(S.size + (S.size >> 2) + 4096, np.uint8)
+    k = _lib.kolmo_klzma_compress(S.ctypes.data, S.size, out.ctypes.data, out.size, 1)
+    t2 = time.perf_counter()
+    return text, (rc if rc > 0 else -1), (int(k) if k > 0 else -1), t1 - t0, t2 - t1
+
+
+def main_slices(a):
+    d = dict(np.load(a.out))
+    jobs = list(zip(d["kind"].tolist(), d["seed"].tolist(), d["n"].tolist()))
+    res = []
+    t0 = time.time()
+    with Pool(a.procs, initializer=_init) as pool:
+        for i, r in enumerate(pool.imap(_slice, jobs, chunksize=4)):
+            res.append(r)
+            if (i + 1) % 1024 == 0:
+                print(f"{i + 1}/{len(jobs)} blocks, {time.time() - t0:.0f} s", flush=True)
+    res = np.array(res, dtype=np.float64)
+    d.update(text=res[:, 0].astype(np.int8), slice_bwt=res[:, 1].astype(np.int64),
+             slice_klzma1=res[:, 2].astype(np.int64), t_slice_bwt=res[:, 3].astype(np.float32),
+             t_slice_klzma1=res[:, 4].astype(np.float32))
+    np.savez_compressed(a.out, **d)
+    print("wrote", a.out, f"{time.time() - t0:.0f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--count", type=int, default=8192)
+    ap.add_argument("--procs", type=int, default=8)
+    ap.add_argument("--out", default=os.path.join(HERE, "router_labels.npz"))
+    ap.add_argument("--slices", action="store_true")
+    a = ap.parse_args()
+    if a.slices:
+        return main_slices(a)
+    import kolmo_amd as K
+    sizes, kinds, seeds = K.batch_plan(ROUTER_SEED, a.count, ROUTER_LO, ROUTER_HI, ROUTER_KINDS)
+    jobs = list(zip(kinds.tolist(), seeds.tolist(), sizes.tolist()))
+    res = []
+    t0 = time.time()
+    with Pool(a.procs, initializer=_init) as pool:
+        for i, r in enumerate(pool.imap(_label, jobs, chunksize=4)):
+            res.append(r)
+            if (i + 1) % 256 == 0:
+                print(f"{i + 1}/{len(jobs)} blocks, {time.time() - t0:.0f} s", flush=True)
+    res = np.array(res, dtype=np.float64)
+    np.savez_compressed(a.out, kind=kinds.astype(np.int32), seed=seeds.astype(np.uint64), n=sizes.astype(np.int64),
+                        size_bwt=res[:, 0].astype(np.int64), size_klzma=res[:, 1].astype(np.int64),
+                        t_bwt=res[:, 2].astype(np.float32), t_klzma=res[:, 3].astype(np.float32))
+    print("wrote", a.out, f"{time.time() - t0:.0f} s")
+
+
+if __name__ == "__main__":
+    main()
