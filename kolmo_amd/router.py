@@ -131,8 +131,10 @@ def outcome_sizes(n, size_bwt, size_klzma):
 
 
 def labels(sizes):
-    """Class of the smallest output (ties go to the lower class: block sorting, then KLZMA, then store)."""
-    return np.argmin(sizes, axis=1).astype(np.int32)
+    """Class of the smallest output; ties go to the cheaper action: storing, then block sorting, then KLZMA (a block
+    neither coder can shrink is stored)."""
+    order = np.array([STORE, BWT, KLZMA])
+    return order[np.argmin(sizes[:, order], axis=1)].astype(np.int32)
 
 
 def weights(sizes):

@@ -124,7 +124,7 @@ def test_labels_weights_decisions():
     n = np.array([1000, 1000, 1000, 1000])
     sizes = R.outcome_sizes(n, [500, 700, -1, 300], [600, 650, -1, 300])
     assert sizes.tolist() == [[500, 600, 1000], [700, 650, 1000], [1000, 1000, 1000], [300, 300, 1000]]
-    assert R.labels(sizes).tolist() == [0, 1, 0, 0]
+    assert R.labels(sizes).tolist() == [0, 1, 2, 0]
     w = R.weights(sizes)
     assert abs(w.mean() - 1) < 1e-6 and w[0] > w[1]   # 0: choosing wrong costs 100 or 500 bytes, 1: 50 or 350
     probs = np.array([[0.9, 0.05, 0.05], [0.4, 0.35, 0.25], [0.2, 0.2, 0.6], [0.1, 0.8, 0.1]])
