@@ -4,7 +4,7 @@
 A step is one forward BWT (suffix sort, BWT bytes, primary and sampled indexes) plus one inverse BWT of the workload,
 already resident in HBM, through the library's device-resident entry points. Output: one JSON line.
 
-  python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload cfg2|cfg1|cfg3|cfg4] [--no-cpu]
+  python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload cfg2|cfg1|cfg3|cfg4] [--no-cpu] [--no-extras]
 
 Workloads (sampling interval r by codec 18's rule for every block):
   cfg2 (default)  one 32 MiB block (seed 2): 70% order-3 Markov text over 96 symbols (Dirichlet(0.2) rows) and 30%
@@ -17,6 +17,9 @@ Workloads (sampling interval r by codec 18's rule for every block):
 Besides throughput it reports, for the dominant kernel, the algorithmic bytes per launch over its average duration
 measured with HIP events on its own stream inside the timed region (roofline), and kolmo's own CPU path
 (oracle/_ref, compiled from kolmo's sources) timed on this host's cores on a bounded sample of the same workload.
+The default cfg2 line also carries, under config: the batched form on cfg4 against one call per block ("batched"),
+the shipped router on the held-out router blocks ("router"), and the cfg2 block through the host-pointer entry
+points, PCIe included ("pcie_inclusive"); --no-extras leaves them out.
 """
 import argparse
 import json
@@ -321,6 +324,52 @@ def run_batch(args, K):
     return value, ms_step, config, roofline, path, cpu
 
 
+def pcie_inclusive(K, reps=3):
+    """The cfg2 block through the host-pointer entry points (copies over PCIe included), best of reps."""
+    gen, seed, n, _ = WORKLOADS["cfg2"]
+    T = K.generate(gen, seed, n)
+    r = 1 << K.codec18_rlog(n)
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        U, I = K.bwt_aux(T, r)
+        K.unbwt_aux(U, r, I)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    log(f"bench: host-pointer forward + inverse {n / best / 2**20:.1f} MiB/s")
+    return {"value": round(n / best / 2**20, 2), "unit": "MiB/s", "ms_per_step": round(best * 1e3, 3),
+            "what": "kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux on host buffers, best of 3"}
+
+
+def router_summary():
+    """The shipped router on the held-out fifth of the router workload (8192 blocks of 256 KiB-4 MiB, seed 5): its
+    bytes against exhaustive best-per-block and its coder throughput, from the committed labels and GPU features."""
+    import numpy as np
+    from kolmo_amd import router as R
+    lab_p = os.path.join(ROOT, "tests", "golden", "router_labels.npz")
+    feat_p = os.path.join(ROOT, "tests", "golden", "router_features.npz")
+    if not (os.path.exists(lab_p) and os.path.exists(feat_p) and os.path.exists(R.MODEL)):
+        return None
+    lab = np.load(lab_p)
+    fz = np.load(feat_p)
+    F = fz["features"]
+    P, th = R.load_model()
+    sizes = R.outcome_sizes(lab["n"], lab["size_bwt"], lab["size_klzma"])
+    test = np.arange(len(F)) % 5 == 4
+    tb, tk = lab["t_bwt"].astype(np.float64), lab["t_klzma"].astype(np.float64)
+    t0 = time.perf_counter()
+    dec = R.decide(R.predict(F[test], P), th)
+    t_pred = time.perf_counter() - t0
+    share = float(fz["gpu_seconds"]) * lab["n"][test].sum() / lab["n"].sum()
+    ev = R.evaluate(dec, sizes[test], tb[test], tk[test], t_features=[share + t_pred])
+    ex = R.evaluate(np.full(test.sum(), -1), sizes[test], tb[test], tk[test])
+    log(f"bench: router {ev['loss_pct']:+.3f}% bytes at {ex['seconds'] / ev['seconds']:.2f}x exhaustive throughput")
+    return {"blocks": int(test.sum()), "size_loss_pct": round(ev["loss_pct"], 4),
+            "throughput_over_exhaustive": round(ex["seconds"] / ev["seconds"], 3),
+            "fallback_blocks": ev["fallback_blocks"],
+            "what": "held-out fifth of 8192 blocks; coder times are single-thread CPU times of kolmo's coders"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -330,6 +379,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run one plain device copy of this many bytes (the reference for byte counters)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="cfg2 only: skip the batched comparison, the router summary and the PCIe-inclusive rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): blocks are independent and a multi-GPU "
@@ -340,8 +391,17 @@ def main():
         raise SystemExit("no usable gfx950 device (or libkbwt_hip.so missing)")
     run = run_batch if args.workload in BATCH else run_single
     value, ms_step, config, roofline, path, cpu = run(args, K)
+    if args.workload == "cfg2" and not args.no_extras:
+        config["pcie_inclusive"] = pcie_inclusive(K)
+        bargs = argparse.Namespace(**vars(args))
+        bargs.workload, bargs.steps, bargs.warmup, bargs.no_cpu = "cfg4", 2, 1, True
+        bv, bms, bcfg, _, _, _ = run_batch(bargs, K)
+        config["batched"] = {"workload": bcfg["workload"], "value": round(bv, 2), "unit": "MiB/s",
+                             "ms_per_step": round(bms, 3), "one_call_per_block": bcfg["one_call_per_block"],
+                             "batched_over_per_block": bcfg["batched_over_per_block"]}
+        config["router"] = router_summary()
     out = {
-        "metric": "fwd+inv BWT MiB/s on 1 MI355X", "value": round(value, 2), "unit": "MiB/s", "n_gpus": 1,
+        "metric": "fwd+inv BWT MiB/s on 1 MI355X: batched vs plain GPU vs host CPU; router size loss %", "value": round(value, 2), "unit": "MiB/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic", "config": config,
         "roofline": roofline, "path_roofline": path, "cpu_baseline": cpu,
