@@ -55,6 +55,7 @@ KERNEL_BYTES = {
     "seg_sort": 18.0,      # read key 4 + suffix 4, write SA 4, ISA 4, row byte 1, read T[i-1] 1
     "isa_part": 16.0,      # read and write one (suffix, head) pair
     "isa_scatter": 12.0,   # read a pair, write ISA
+    "emit_compact": 0.0,   # packs the next round's group list (a few bytes per group, not per row)
 }
 
 WORKLOADS = {

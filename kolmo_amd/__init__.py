@@ -335,3 +335,46 @@ WORKLOADS = {
     "cfg2_mixed_32MiB": ("mixed", 2, 32 << 20),
     "cfg3_repeats_128MiB": ("repeats", 3, 128 << 20),
 }
+
+
+# ---- autotuner (kbwt_autotune): launch parameters per size class, none of which changes any output ----
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("sort_cfg", ctypes.c_int32), ("inv_dshift", ctypes.c_int32), ("seg_rank", ctypes.c_int32),
+                ("ms", ctypes.c_float)]
+
+
+DEFAULT_TUNING = (0, 6, 16)
+
+
+def _tune_sigs(L):
+    L.kbwt_autotune.restype = ctypes.c_int32
+    L.kbwt_autotune.argtypes = [ctypes.c_void_p, ctypes.POINTER(Seg), ctypes.c_int32, ctypes.c_void_p,
+                                ctypes.POINTER(Tuning), ctypes.POINTER(Tuning)]
+    L.kbwt_set_tuning.restype = ctypes.c_int32
+    L.kbwt_set_tuning.argtypes = [ctypes.c_int32, ctypes.POINTER(Tuning)]
+    L.kbwt_get_tuning.restype = ctypes.c_int32
+    L.kbwt_get_tuning.argtypes = [ctypes.c_int32, ctypes.POINTER(Tuning)]
+    return L
+
+
+def autotune(dT, segs, count, start=None, stream=None):
+    """Time the candidate launch parameters on these device blocks and install the fastest for their size class.
+    Returns (best (sort_cfg, inv_dshift, seg_rank), best ms, ms at the start point)."""
+    import torch
+    L = _tune_sigs(lib())
+    best = Tuning()
+    st = Tuning(*(start or DEFAULT_TUNING), 0.0)
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _check(L.kbwt_autotune(dT.data_ptr(), segs, count, s, ctypes.byref(best), ctypes.byref(st)))
+    return (best.sort_cfg, best.inv_dshift, best.seg_rank), best.ms, st.ms
+
+
+def set_tuning(t, size_log2=-1):
+    _check(_tune_sigs(lib()).kbwt_set_tuning(size_log2, ctypes.byref(Tuning(*t, 0.0))))
+
+
+def get_tuning(size_log2):
+    t = Tuning()
+    _check(_tune_sigs(lib()).kbwt_get_tuning(size_log2, ctypes.byref(t)))
+    return (t.sort_cfg, t.inv_dshift, t.seg_rank)

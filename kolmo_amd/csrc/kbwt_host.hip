@@ -53,7 +53,7 @@ struct Workspace {
     std::mutex mu;
     uint32_t epoch = 1;
     Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, st_max, samples, stage, blk, counts, dbase,
-        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, lo, tstart, act, rfeat, bigram, rmeta;
+        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es, ecnt, eoff;
     uint32_t *counters = nullptr;           // 16 tile counters (zeroed once)
     uint32_t *gcnt = nullptr;               // 4 group-list counters
     uint32_t *h_gcnt = nullptr;
@@ -212,6 +212,28 @@ uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi) {
     return x < (long)lo ? lo : (x > (long)hi ? hi : (uint32_t)x);
 }
 
+// Launch parameters per size class (ceil(log2 N) of a call's packed length), set by kbwt_autotune or
+// kbwt_set_tuning. An environment variable (KBWT_SORT_CFG, KBWT_INV_DSHIFT, KBWT_SEG_RANK) overrides the table.
+struct Tuning {
+    uint32_t sort_cfg = 0, inv_dshift = 6, seg_rank = SEG_RANK_MAX;
+};
+constexpr int TUNE_CLASSES = 33;
+std::mutex g_tune_mu;
+Tuning g_tune[TUNE_CLASSES];
+uint32_t size_class(uint32_t N) {
+    uint32_t c = 0;
+    while (c < 32 && ((uint64_t)1 << c) < N) c++;
+    return c;
+}
+Tuning tuning_for(uint32_t N) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    Tuning t = g_tune[size_class(N)];
+    t.sort_cfg = env_u32("KBWT_SORT_CFG", t.sort_cfg, 0, SORT_NCFG - 1);
+    t.inv_dshift = env_u32("KBWT_INV_DSHIFT", t.inv_dshift, 2, 12);
+    t.seg_rank = env_u32("KBWT_SEG_RANK", t.seg_rank, 0, SEG_TILE);
+    return t;
+}
+
 // first[b] = tiles of the blocks before b, first[B] = all tiles.
 template <class F> uint32_t fill_tiles(uint32_t *first, uint32_t B, F tiles_of) {
     uint32_t t = 0;
@@ -270,7 +292,8 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     memset(&st, 0, sizeof st);
     st.n = (int32_t)N;
     st.blocks = (int32_t)B;
-    const int scfg = (int)env_u32("KBWT_SORT_CFG", 0, 0, SORT_NCFG - 1);
+    const Tuning tune = tuning_for(N);
+    const int scfg = (int)tune.sort_cfg;
     const uint32_t stile = sort_tile(scfg);
     const size_t NN = std::max<size_t>(N, 64);
     grow(w.K0, NN * 8);
@@ -439,10 +462,15 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     }
     grow(w.go, GMAX * 4);
     grow(w.tstart, (NN / SEG_TILE + 2) * 4);
+    grow(w.eh, (NN / SEG_TILE + 2) * SEG_TILE * 4);
+    grow(w.es, (NN / SEG_TILE + 2) * SEG_TILE * 4);
+    grow(w.ecnt, (NN / SEG_TILE + 2) * 4);
+    grow(w.eoff, (NN / SEG_TILE + 2) * 4);
     grow(w.sec, NN * 4);
+    grow(w.sufl, NN * 4);
     grow(w.lo, GMAX * 8);
     uint32_t *gcnt = w.gcnt;          // [small groups, large groups, small rows] of the list being built
-    const uint32_t rank_max = env_u32("KBWT_SEG_RANK", SEG_RANK_MAX, 0, SEG_TILE);
+    const uint32_t rank_max = tune.seg_rank;
     std::vector<uint32_t> h_lh, h_ls, h_lo;
     auto read_lists = [&](int c, uint32_t &Gs, uint32_t &Gl) {
         HIPCHK(hipMemcpyAsync(w.h_gcnt, gcnt, 16, hipMemcpyDeviceToHost, s));
@@ -498,15 +526,16 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         sa.SA = SA;
         sa.ISA = ISA;
         sa.sec = P<uint32_t>(w.sec);
+        sa.sufl = P<uint32_t>(w.sufl);
         sa.L = L;
         sa.T = dT;
         sa.blk = blk;
         sa.blocks = d_bl;
         sa.n = N;
         sa.round = (uint32_t)round;
-        sa.nh = P<uint32_t>(w.gh[cur ^ 1]);
-        sa.ns = P<uint32_t>(w.gs[cur ^ 1]);
-        sa.ncnt = gcnt;
+        sa.nh = P<uint32_t>(w.eh);
+        sa.ns = P<uint32_t>(w.es);
+        sa.ncnt = P<uint32_t>(w.ecnt);
         sa.rank_max = rank_max;
         if (Gs) TIMED("seg_gather", Es, launch_seg_round(0, sa, s));
         Shifts sh{};
@@ -538,7 +567,17 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         }
         // sort and split; the next round's lists are built from zero
         zero_counts();
-        if (Gs) TIMED("seg_sort", Es, launch_seg_round(1, sa, s));
+        if (Gs) {
+            TIMED("seg_sort", Es, launch_seg_round(1, sa, s));
+            // the tiles' emissions into the next small list; its length goes to gcnt[0], where the large groups'
+            // subgroups are appended after it
+            const uint32_t nt = (Es + SEG_TILE - 1) / SEG_TILE;
+            TIMED("emit_compact", Es,
+                  launch_scan_groups(P<uint32_t>(w.ecnt), nt, P<uint32_t>(w.eoff), gcnt, nullptr,
+                                     P<uint64_t>(w.st_sum), w.counters + 3, w.epoch++, s);
+                  launch_compact_emits(P<uint32_t>(w.eh), P<uint32_t>(w.es), P<uint32_t>(w.ecnt), P<uint32_t>(w.eoff),
+                                       nt, P<uint32_t>(w.gh[cur ^ 1]), P<uint32_t>(w.gs[cur ^ 1]), s));
+        }
         LAUNCHCHK();
         if (Gl) {
             int src = 0;
@@ -627,7 +666,7 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     st.blocks = (int32_t)B;
     // one splitter row in D = 2^dshift; the first cap bytes of each sublist are staged during the counting walk
     // (the walk's time is its longest sublist, about D * ln(splitters) steps, times the load latency)
-    const uint32_t dshift = env_u32("KBWT_INV_DSHIFT", 6, 2, 12);
+    const uint32_t dshift = tuning_for(N).inv_dshift;
     uint32_t cap = env_u32("KBWT_INV_CAP", 16u << dshift, 1, 1u << 16);
     const uint32_t dmask = (1u << dshift) - 1u;
     uint32_t S = 0;
@@ -1012,6 +1051,126 @@ int32_t kbwt_dev_unbwt_aux_batch(const uint8_t *dT, uint8_t *dU, const kbwt_seg 
     });
 }
 
+// ---- autotuner: coordinate descent over the launch parameters, timed on the caller's own blocks
+
+static bool tuning_ok(const kbwt_tuning *t) {
+    return t && t->sort_cfg >= 0 && t->sort_cfg < SORT_NCFG && t->inv_dshift >= 2 && t->inv_dshift <= 12 &&
+           t->seg_rank >= 0 && t->seg_rank <= SEG_TILE;
+}
+
+int32_t kbwt_set_tuning(int32_t size_log2, const kbwt_tuning *t) {
+    if (!tuning_ok(t) || size_log2 >= TUNE_CLASSES) return KBWT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (int c = 0; c < TUNE_CLASSES; c++)
+        if (size_log2 < 0 || c == size_log2) {
+            g_tune[c].sort_cfg = (uint32_t)t->sort_cfg;
+            g_tune[c].inv_dshift = (uint32_t)t->inv_dshift;
+            g_tune[c].seg_rank = (uint32_t)t->seg_rank;
+        }
+    return 0;
+}
+
+int32_t kbwt_get_tuning(int32_t size_log2, kbwt_tuning *t) {
+    if (t == NULL || size_log2 < 0 || size_log2 >= TUNE_CLASSES) return KBWT_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    t->sort_cfg = (int32_t)g_tune[size_log2].sort_cfg;
+    t->inv_dshift = (int32_t)g_tune[size_log2].inv_dshift;
+    t->seg_rank = (int32_t)g_tune[size_log2].seg_rank;
+    t->ms = 0;
+    return 0;
+}
+
+int32_t kbwt_autotune(const uint8_t *dT, const kbwt_seg *segs, int32_t count, void *stream, kbwt_tuning *best,
+                      kbwt_tuning *start) {
+    if (dT == NULL || best == NULL || !batch_ok(segs, count, true) || count == 0) return -1;
+    int64_t total = 0;
+    for (int32_t b = 0; b < count; b++) total += segs[b].n;
+    if (total < 2) return -1;
+    const uint32_t cls = size_class((uint32_t)total);
+    return guarded([&] {
+        Workspace &w = workspace();
+        std::lock_guard<std::mutex> lk(w.mu);
+        hipStream_t s = stream ? (hipStream_t)stream : w.stream;
+        const int64_t span = segs[count - 1].offset + segs[count - 1].n;
+        void *dU = nullptr, *dB = nullptr;
+        HIPCHK(hipMalloc(&dU, (size_t)span + 64));
+        HIPCHK(hipMalloc(&dB, (size_t)span + 64));
+        std::vector<int32_t> I;
+        size_t ns = 0;
+        for (int32_t b = 0; b < count; b++) ns += nsamples(segs[b].n, segs[b].r);
+        I.resize(ns);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        // one forward + inverse with tuning t installed for this size class; best of two timed runs after a warm-up
+        auto trial = [&](const Tuning &t) {
+            {
+                std::lock_guard<std::mutex> tl(g_tune_mu);
+                g_tune[cls] = t;
+            }
+            float best_ms = 1e30f;
+            for (int rep = 0; rep < 3; rep++) {
+                HIPCHK(hipEventRecord(e0, s));
+                forward_batch_dev(w, dT, (uint8_t *)dU, segs, count, I.data(), s);
+                inverse_batch_dev(w, (const uint8_t *)dU, (uint8_t *)dB, segs, count, I.data(), s);
+                HIPCHK(hipEventRecord(e1, s));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0 && ms < best_ms) best_ms = ms;
+            }
+            return best_ms;
+        };
+        Tuning cur;
+        if (start && tuning_ok(start)) {
+            cur.sort_cfg = (uint32_t)start->sort_cfg;
+            cur.inv_dshift = (uint32_t)start->inv_dshift;
+            cur.seg_rank = (uint32_t)start->seg_rank;
+        }
+        float cur_ms = trial(cur);
+        const float start_ms = cur_ms;
+        const uint32_t dshifts[] = {4, 5, 6, 7}, ranks[] = {0, 8, 16, 64};
+        for (int pass = 0; pass < 2; pass++) {
+            bool moved = false;
+            for (uint32_t c = 0; c < (uint32_t)SORT_NCFG; c++) {
+                if (c == cur.sort_cfg) continue;
+                Tuning t = cur;
+                t.sort_cfg = c;
+                const float ms = trial(t);
+                if (ms < cur_ms) { cur = t; cur_ms = ms; moved = true; }
+            }
+            for (uint32_t d : dshifts) {
+                if (d == cur.inv_dshift) continue;
+                Tuning t = cur;
+                t.inv_dshift = d;
+                const float ms = trial(t);
+                if (ms < cur_ms) { cur = t; cur_ms = ms; moved = true; }
+            }
+            for (uint32_t r : ranks) {
+                if (r == cur.seg_rank) continue;
+                Tuning t = cur;
+                t.seg_rank = r;
+                const float ms = trial(t);
+                if (ms < cur_ms) { cur = t; cur_ms = ms; moved = true; }
+            }
+            if (!moved) break;
+        }
+        {
+            std::lock_guard<std::mutex> tl(g_tune_mu);
+            g_tune[cls] = cur;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(dU);
+        (void)hipFree(dB);
+        best->sort_cfg = (int32_t)cur.sort_cfg;
+        best->inv_dshift = (int32_t)cur.inv_dshift;
+        best->seg_rank = (int32_t)cur.seg_rank;
+        best->ms = cur_ms;
+        if (start) start->ms = start_ms;
+    });
+}
+
 // Host-pointer batches: the span [first offset, last end) goes to the device once, keeping the offsets.
 int32_t kbwt_bwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs, int32_t count, int32_t *I) {
     if (T == NULL || U == NULL || I == NULL || !batch_ok(segs, count, true)) return -1;
@@ -1124,7 +1283,7 @@ void kbwt_release(void) {
                    &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.st_max, &w.samples,
                    &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
                    &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
-                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta};
+                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,  &w.sufl,     &w.eh,     &w.es,     &w.ecnt,   &w.eoff};
     for (Buf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;

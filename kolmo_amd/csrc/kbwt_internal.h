@@ -94,13 +94,14 @@ struct SegRoundArgs {
     const uint32_t *gh, *go;      // the small groups: heads and first-row offsets (prefix sums of sizes)
     const uint32_t *tstart;       // first group of every SEG_TILE-row tile, and G at the end
     uint32_t G, E;                // groups, rows
-    uint32_t *SA, *ISA, *sec;
+    uint32_t *SA, *ISA, *sec;     // sec: each row's key, in list order (row go[g] + k of group g)
+    uint32_t *sufl;               // each row's suffix, in list order
     uint8_t *L;
     const uint8_t *T;
     const uint16_t *blk;
     const BlockInfo *blocks;      // every block (blocks[0] when blk == nullptr)
     uint32_t n, round;            // doubling round >= 1: offset round_offset(block, round)
-    uint32_t *nh, *ns, *ncnt;     // next round's small list and its counter
+    uint32_t *nh, *ns, *ncnt;     // the split's emissions: per tile, SEG_TILE entries at tile * SEG_TILE, and counts
     uint32_t rank_max;            // tiles whose groups have at most rank_max rows rank them directly
 };
 
@@ -108,6 +109,8 @@ void launch_elems_to_groups(const uint32_t *pos, const uint32_t *grp, const uint
                             uint32_t *sh, uint32_t *ss, uint32_t *lh, uint32_t *ls, uint32_t *cnt, hipStream_t s);
 void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *tot, uint32_t *tstart,
                         uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s);
+void launch_compact_emits(const uint32_t *eh, const uint32_t *es, const uint32_t *ecnt, const uint32_t *eoff,
+                          uint32_t ntiles, uint32_t *nh, uint32_t *ns, hipStream_t s);
 void launch_large_elems(const uint32_t *lh, const uint32_t *lo, uint32_t G, uint32_t m, uint32_t *pos, uint32_t *grp,
                         hipStream_t s);
 void launch_seg_round(int mode, const SegRoundArgs &a, hipStream_t s);

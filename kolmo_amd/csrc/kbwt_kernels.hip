@@ -98,16 +98,21 @@ __global__ __launch_bounds__(256) void k_byte_hist(const uint8_t *__restrict__ T
             if (cur >= 0) flush();
             cur = (int)r.b;
         }
-        if (r.end - r.beg == (size_t)HIST_TILE && (r.beg & 15) == 0) {   // 16 bytes per thread
-            const uint4 v = reinterpret_cast<const uint4 *>(T + r.beg)[tid];
+        // 16-byte loads over the tile's aligned middle, single bytes for the unaligned head and tail
+        const size_t mis = (size_t)(reinterpret_cast<uintptr_t>(T + r.beg) & 15u);
+        const size_t ab = r.beg + (mis ? 16 - mis : 0) < r.end ? r.beg + (mis ? 16 - mis : 0) : r.end;
+        const size_t ae = ab + ((r.end - ab) & ~(size_t)15);
+        const uint4 *vp = reinterpret_cast<const uint4 *>(T + ab);   // 16-byte aligned address
+        for (size_t q16 = tid; q16 < ((ae - ab) >> 4); q16 += 256) {
+            const uint4 v = vp[q16];
             const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
             for (int q = 0; q < 4; q++)
 #pragma unroll
                 for (int k = 0; k < 4; k++) atomicAdd(&h[cp][(wd[q] >> (8 * k)) & 255u], 1u);
-        } else {
-            for (size_t e = r.beg + tid; e < r.end; e += 256) atomicAdd(&h[cp][T[e]], 1u);
         }
+        if ((size_t)tid < ab - r.beg) atomicAdd(&h[cp][T[r.beg + tid]], 1u);
+        if ((size_t)tid < r.end - ae) atomicAdd(&h[cp][T[ae + tid]], 1u);
     }
     if (cur >= 0) flush();
 }
@@ -378,94 +383,141 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
                                                    uint32_t *__restrict__ grp_out, uint32_t *__restrict__ m_out,
                                                    uint64_t *__restrict__ st_sum, uint64_t *__restrict__ st_max,
                                                    uint32_t *__restrict__ counter, uint32_t epoch, uint32_t ntiles) {
-    constexpr int TILE = BLOCK * ITEMS;
+    // Rows move between HBM and the threads through LDS: coalesced loads and stores in row order, and each thread
+    // works on ITEMS consecutive rows, so one block-wide scan covers the tile. The LDS row of thread t starts at
+    // t * (ITEMS + 1): the pad keeps a wave's lanes on distinct banks.
+    constexpr int TILE = BLOCK * ITEMS, PITCH = ITEMS + 1;
+    __shared__ uint64_t s_k[BLOCK * PITCH];      // keys, later the (suffix, head) pairs
+    __shared__ uint32_t s_v[BLOCK * PITCH];      // suffixes
+    __shared__ uint32_t s_cp[TILE], s_cg[TILE];  // the tile's compacted rows (position, head)
+    __shared__ uint16_t s_b[BLOCK * PITCH];       // block ids (batches)
     __shared__ uint32_t s_scratch[32];
+    __shared__ uint32_t s_imax[BLOCK];
     __shared__ uint32_t s_tile, s_ex_sum, s_ex_max;
     const int tid = threadIdx.x;
     if (tid == 0) s_tile = next_tile(counter, ntiles);
     __syncthreads();
     const uint32_t tile = s_tile;
-    const size_t base = (size_t)tile * TILE;
-
-    uint32_t lkeep[ITEMS], lhead[ITEMS], lp[ITEMS];
-    bool lkept[ITEMS];
-    uint8_t lbyte[ITEMS];
-    uint32_t run_sum = 0, run_max = 0;
+    const size_t t0 = (size_t)tile * TILE;
+    const uint32_t cnt = (uint32_t)(t0 + TILE <= m ? TILE : m - t0);
+    auto at = [](uint32_t j) { return (j / ITEMS) * PITCH + j % ITEMS; };
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        size_t a = base + (size_t)k * BLOCK + tid;
-        bool valid = a < m;
-        bool flag = false, keep = false;
-        uint32_t p = 0;
-        lbyte[k] = 0;
-        if (valid) {
-            uint64_t kr = K[a];
-            uint64_t kc = kr >> kshift;
-            flag = (a == 0) || ((K[a - 1] >> kshift) != kc);
-            bool flagn = (a + 1 >= m) || ((K[a + 1] >> kshift) != kc);
-            if (blk) {
-                uint16_t bc = blk[a];
-                flag = flag || (a > 0 && blk[a - 1] != bc);
-                flagn = flagn || (a + 1 < m && blk[a + 1] != bc);
-            }
-            keep = !(flag && flagn);
-            p = pos ? pos[a] : (uint32_t)a;
-            lbyte[k] = (uint8_t)kr;
+        const uint32_t j = k * BLOCK + tid;
+        if (j < cnt) {
+            s_k[at(j)] = K[t0 + j];
+            s_v[at(j)] = V[t0 + j];
+            if (blk) s_b[at(j)] = blk[t0 + j];
         }
-        uint32_t ex_s, in_m, tot_s, tot_m;
-        block_scan_sum_max<BLOCK>(keep ? 1u : 0u, flag ? p : 0u, ex_s, in_m, tot_s, tot_m, s_scratch);
-        lkeep[k] = run_sum + ex_s;
-        lhead[k] = run_max > in_m ? run_max : in_m;
-        lkept[k] = keep;
-        lp[k] = p;
-        run_sum += tot_s;
-        run_max = run_max > tot_m ? run_max : tot_m;
     }
+    // the rows on either side of the tile
+    const bool has_prev = t0 > 0, has_next = t0 + cnt < m;
+    const uint64_t kprev_t = has_prev ? K[t0 - 1] >> kshift : 0, knext_t = has_next ? K[t0 + cnt] >> kshift : 0;
+    const uint16_t bprev_t = has_prev && blk ? blk[t0 - 1] : 0, bnext_t = has_next && blk ? blk[t0 + cnt] : 0;
+    __syncthreads();
+
+    const uint32_t r0 = (uint32_t)tid * ITEMS;   // this thread's first row in the tile
+    uint64_t kc[ITEMS + 2];                      // [0]: row r0 - 1, [ITEMS + 1]: row r0 + ITEMS
+    uint16_t bk[ITEMS + 2];
+    uint32_t lp[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS + 2; k++) {
+        const int32_t j = (int32_t)r0 + k - 1;
+        if (j < 0) { kc[k] = kprev_t; bk[k] = bprev_t; }
+        else if ((uint32_t)j >= cnt) { kc[k] = knext_t; bk[k] = bnext_t; }
+        else { kc[k] = s_k[at((uint32_t)j)] >> kshift; bk[k] = blk ? s_b[at((uint32_t)j)] : 0; }
+    }
+    uint8_t lbyte[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const uint32_t j = r0 + k;
+        lbyte[k] = j < cnt ? (uint8_t)s_k[at(j)] : 0;
+        lp[k] = j < cnt ? (pos ? pos[t0 + j] : (uint32_t)(t0 + j)) : 0u;
+    }
+    uint32_t flags = 0, keeps = 0;               // bit k: row k starts a group / stays active
+    uint32_t nkeep = 0, tmax = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const uint32_t j = r0 + k;
+        if (j >= cnt) continue;
+        const bool pv = j > 0 || has_prev, nx = j + 1 < cnt || has_next;
+        const bool flag = !pv || kc[k] != kc[k + 1] || bk[k] != bk[k + 1];
+        const bool flagn = !nx || kc[k + 2] != kc[k + 1] || bk[k + 2] != bk[k + 1];
+        if (flag) { flags |= 1u << k; tmax = lp[k]; }   // SA positions increase along the rows
+        if (!(flag && flagn)) { keeps |= 1u << k; nkeep++; }
+    }
+    uint32_t ex_s, in_m, tot_s, tot_m;
+    block_scan_sum_max<BLOCK>(nkeep, tmax, ex_s, in_m, tot_s, tot_m, s_scratch);
+    s_imax[tid] = in_m;
     // wave 0 looks back over the compaction counts, wave 1 over the group heads
     const int wave = tid / WAVE;
     if (wave == 0) {
-        uint32_t ex_s = 0;
+        uint32_t ex = 0;
         if (tile == 0) {
-            if (tid == 0) st_store(st_sum, st_pack(epoch, ST_INC, run_sum));
+            if (tid == 0) st_store(st_sum, st_pack(epoch, ST_INC, tot_s));
         } else {
-            if (tid == 0) st_store(st_sum + tile, st_pack(epoch, ST_AGG, run_sum));
-            ex_s = lookback_wave<false>(st_sum, tile, epoch);
-            if (tid == 0) st_store(st_sum + tile, st_pack(epoch, ST_INC, ex_s + run_sum));
+            if (tid == 0) st_store(st_sum + tile, st_pack(epoch, ST_AGG, tot_s));
+            ex = lookback_wave<false>(st_sum, tile, epoch);
+            if (tid == 0) st_store(st_sum + tile, st_pack(epoch, ST_INC, ex + tot_s));
         }
         if (tid == 0) {
-            s_ex_sum = ex_s;
-            if (tile == ntiles - 1) *m_out = ex_s + run_sum;
+            s_ex_sum = ex;
+            if (tile == ntiles - 1) *m_out = ex + tot_s;
         }
     } else if (wave == 1) {
         uint32_t ex_m = 0;
         if (tile == 0) {
-            if (tid == WAVE) st_store(st_max, st_pack(epoch, ST_INC, run_max));
+            if (tid == WAVE) st_store(st_max, st_pack(epoch, ST_INC, tot_m));
         } else {
-            if (tid == WAVE) st_store(st_max + tile, st_pack(epoch, ST_AGG, run_max));
+            if (tid == WAVE) st_store(st_max + tile, st_pack(epoch, ST_AGG, tot_m));
             ex_m = lookback_wave<true>(st_max, tile, epoch);
-            uint32_t im = ex_m > run_max ? ex_m : run_max;
+            const uint32_t im = ex_m > tot_m ? ex_m : tot_m;
             if (tid == WAVE) st_store(st_max + tile, st_pack(epoch, ST_INC, im));
         }
         if (tid == WAVE) s_ex_max = ex_m;
     }
     __syncthreads();
-    const uint32_t ex_sum = s_ex_sum, ex_max = s_ex_max;
+    // the head before this thread's first row: the tile's prefix and the threads before this one
+    uint32_t head = s_ex_max;
+    if (tid > 0 && s_imax[tid - 1] > head) head = s_imax[tid - 1];
+    uint32_t o = ex_s;                            // compacted rows before this thread's, within the tile
+    uint64_t lbytes = 0;
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        size_t a = base + (size_t)k * BLOCK + tid;
-        if (a < m) {
-            uint32_t head = ex_max > lhead[k] ? ex_max : lhead[k];
-            uint32_t i = V[a];
-            if (SA) SA[lp[k]] = i;
-            if (pairs) pairs[a] = ((uint64_t)i << 32) | head;
-            else ISA[i] = head;
-            L[lp[k]] = T ? (i ? T[i - 1] : (uint8_t)0) : lbyte[k];
-            if (lkept[k]) {
-                uint32_t o = ex_sum + lkeep[k];
-                pos_out[o] = lp[k];
-                grp_out[o] = head;
-            }
+        const uint32_t j = r0 + k;
+        if (j >= cnt) continue;
+        if (flags & (1u << k)) head = lp[k] > head ? lp[k] : head;
+        const uint32_t i = s_v[at(j)];
+        if (SA) SA[lp[k]] = i;
+        if (pairs) s_k[at(j)] = ((uint64_t)i << 32) | head;
+        else ISA[i] = head;
+        const uint8_t c = T ? (i ? T[i - 1] : (uint8_t)0) : lbyte[k];
+        if (pos) L[lp[k]] = c;
+        else lbytes |= (uint64_t)c << (8 * k);
+        if (keeps & (1u << k)) {
+            s_cp[o] = lp[k];
+            s_cg[o] = head;
+            o++;
         }
+    }
+    if (!pos) {   // rows in SA order: the thread's ITEMS bytes of L are consecutive
+        if (r0 + ITEMS <= cnt && ((t0 + r0) & 7) == 0) *reinterpret_cast<uint64_t *>(L + t0 + r0) = lbytes;
+        else
+            for (int k = 0; k < ITEMS; k++)
+                if (r0 + k < cnt) L[t0 + r0 + k] = (uint8_t)(lbytes >> (8 * k));
+    }
+    __syncthreads();
+    // coalesced stores in row order: the pairs, then the compacted rows
+    if (pairs)
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t j = k * BLOCK + tid;
+            if (j < cnt) pairs[t0 + j] = s_k[at(j)];
+        }
+    const uint32_t base_out = s_ex_sum;
+    for (uint32_t j = tid; j < tot_s; j += BLOCK) {
+        pos_out[base_out + j] = s_cp[j];
+        grp_out[base_out + j] = s_cg[j];
     }
 }
 

@@ -129,8 +129,11 @@ __global__ __launch_bounds__(256) void k_scan_groups(const uint32_t *__restrict_
         if (g < G) {
             go[g] = run;
             const uint32_t t = (run + (uint32_t)SEG_TILE - 1) / (uint32_t)SEG_TILE;   // first tile start >= run
-            if ((uint64_t)t * SEG_TILE < (uint64_t)run + v[k]) tstart[t] = (uint64_t)t * SEG_TILE == run ? (uint32_t)g : (uint32_t)g + 1;
-            if (g + 1 == G) tstart[(run + v[k] + (uint32_t)SEG_TILE - 1) / (uint32_t)SEG_TILE] = G;
+            if (tstart) {
+                if ((uint64_t)t * SEG_TILE < (uint64_t)run + v[k])
+                    tstart[t] = (uint64_t)t * SEG_TILE == run ? (uint32_t)g : (uint32_t)g + 1;
+                if (g + 1 == G) tstart[(run + v[k] + (uint32_t)SEG_TILE - 1) / (uint32_t)SEG_TILE] = G;
+            }
         }
         run += v[k];
     }
@@ -154,13 +157,15 @@ __device__ __forceinline__ uint32_t sec_of(const uint32_t *ISA, uint32_t i, uint
     return j < end ? ISA[j] + h : end - 1u - i;
 }
 
-// One tile of small groups. MODE 0 gathers sec[p] for every row p of the tile's groups; MODE 1 sorts each group by
-// sec in LDS, rewrites SA, ISA and the BWT bytes of its rows, and appends the subgroups of two or more to the next
-// list (nh, ns; *ncnt counts them).
+// One tile of small groups. MODE 0 gathers every row's key; MODE 1 sorts each group by it in LDS, rewrites SA, ISA
+// and the BWT bytes of its rows, and writes the subgroups of two or more into the tile's own area of the emission
+// buffers (nh, ns at tile * SEG_TILE; ncnt[tile] counts them), which k_compact_emits packs into the next list. One
+// shared counter for all tiles would serialise every workgroup on a single atomic.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ gh, const uint32_t *__restrict__ go,
                                                    const uint32_t *__restrict__ tstart, uint32_t G, uint32_t E, uint32_t *__restrict__ SA,
                                                    uint32_t *__restrict__ ISA, uint32_t *__restrict__ sec,
+                                                   uint32_t *__restrict__ sufl,
                                                    uint8_t *__restrict__ L, const uint8_t *__restrict__ T,
                                                    const uint16_t *__restrict__ blk,
                                                    const BlockInfo *__restrict__ blocks, uint32_t n, uint32_t round,
@@ -171,6 +176,7 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     constexpr int ITEMS = CAP / 256;
     constexpr int W = 4;
     __shared__ uint32_t s_gh[MAXG], s_go[MAXG];
+    __shared__ uint32_t s_hoff[MODE ? 1 : MAXG], s_gend[MODE ? 1 : MAXG];   // gather: each group's offset and block end
     __shared__ uint32_t s_g0, s_g1, s_cnt, s_maxsz;
     __shared__ uint64_t s_buf[MODE ? 2 * CAP : 1];
     __shared__ uint32_t s_i[MODE ? CAP : 1];
@@ -178,7 +184,6 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     __shared__ uint32_t s_start[256];
     __shared__ uint32_t s_scratch[16];
     __shared__ uint64_t s_or[W], s_and[W];
-    __shared__ uint32_t s_base;
 
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
     if (tid == 0) {
@@ -188,39 +193,78 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     }
     __syncthreads();
     const uint32_t g0 = s_g0, g1 = s_g1, ng = g1 - g0;
-    if (ng == 0) return;
+    if (ng == 0) {
+        if (MODE == 1 && tid == 0) ncnt[blockIdx.x] = 0;
+        return;
+    }
     const uint32_t span0 = go[g0], span1 = g1 < G ? go[g1] : E;
     const uint32_t cnt = span1 - span0;        // <= CAP - 1
     for (uint32_t j = tid; j < ng; j += 256) {
-        s_gh[j] = gh[g0 + j];
+        const uint32_t h = gh[g0 + j];
+        s_gh[j] = h;
         s_go[j] = go[g0 + j] - span0;
+        if (MODE == 0) {   // a group lies in one block
+            const BlockInfo bi = blocks[blk ? blk[h] : 0u];
+            s_hoff[j] = round_offset(bi, round);
+            s_gend[j] = bi.off + bi.n;
+        }
     }
     __syncthreads();
     if (MODE == 1)
         for (uint32_t j = tid; j < ng; j += 256) atomicMax(&s_maxsz, (j + 1 < ng ? s_go[j + 1] : cnt) - s_go[j]);
 
-    // gather: row e of the tile belongs to group lg (binary search over the tile's group offsets)
-    uint64_t ko = 0, ka = ~0ull;
-    for (uint32_t e = tid; e < cnt; e += 256) {
+    // gather: row e of the tile belongs to group lg (binary search over the tile's group offsets). MODE 0 reads each
+    // row's suffix from SA and its key at i + h, and stores both in list order (row span0 + e of the round: sufl,
+    // sec), so the split reads them back contiguously. The ITEMS rows of a thread are independent, so their loads
+    // are issued together.
+    uint32_t ga[ITEMS], gi[ITEMS], gv[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const uint32_t e = tid + k * 256;
         uint32_t a = 0, b = ng;                // last lg with s_go[lg] <= e
         while (b - a > 1) {
-            uint32_t mid = (a + b) >> 1;
+            const uint32_t mid = (a + b) >> 1;
             if (s_go[mid] <= e) a = mid; else b = mid;
         }
-        const uint32_t p = s_gh[a] + (e - s_go[a]);
-        const uint32_t i = SA[p];
-        if (MODE == 0) {
-            const BlockInfo bi = blocks[blk ? blk[p] : 0u];
-            sec[p] = sec_of(ISA, i, round_offset(bi, round), bi.off + bi.n);
-        } else {
-            const uint64_t key = (((uint64_t)a << 32) | sec[p]) << 12 | e;   // group, sec, row (sorts as a unit)
+        ga[k] = a;
+    }
+    if (MODE == 0) {
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t e = tid + k * 256;
+            gi[k] = e < cnt ? SA[s_gh[ga[k]] + (e - s_go[ga[k]])] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++)
+            if (tid + k * 256 < cnt) gv[k] = sec_of(ISA, gi[k], s_hoff[ga[k]], s_gend[ga[k]]);
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t e = tid + k * 256;
+            if (e < cnt) {
+                sufl[span0 + e] = gi[k];
+                sec[span0 + e] = gv[k];
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const uint32_t e = tid + k * 256;
+        gi[k] = e < cnt ? sufl[span0 + e] : 0u;
+        gv[k] = e < cnt ? sec[span0 + e] : 0u;
+    }
+    uint64_t ko = 0, ka = ~0ull;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const uint32_t e = tid + k * 256;
+        if (e < cnt) {
+            const uint64_t key = (((uint64_t)ga[k] << 32) | gv[k]) << 12 | e;   // group, sec, row (sort as a unit)
             s_buf[e] = key;
-            s_i[e] = i;
+            s_i[e] = gi[k];
             ko |= key;
             ka &= key;
         }
     }
-    if (MODE == 0) return;
     __syncthreads();
 
     // Tiles of small groups only: each row counts, within its group, the rows of smaller sec (less) and of equal sec
@@ -260,9 +304,8 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         }
         uint32_t total_emit;
         const uint32_t my_emit = block_excl_sum<256>(nemit, s_scratch, &total_emit);
-        if (tid == 0) s_base = total_emit ? atomicAdd(ncnt, total_emit) : 0u;
-        __syncthreads();
-        uint32_t at = s_base + my_emit;
+        if (tid == 0) ncnt[blockIdx.x] = total_emit;
+        size_t at = (size_t)blockIdx.x * SEG_TILE + my_emit;
 #pragma unroll
         for (int k = 0; k < PERR; k++)
             if (es[k]) { nh[at] = eh[k]; ns[at] = es[k]; at++; }
@@ -381,9 +424,8 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     }
     uint32_t total_emit;
     const uint32_t my_emit = block_excl_sum<256>(nemit, s_scratch, &total_emit);
-    if (tid == 0) s_base = total_emit ? atomicAdd(ncnt, total_emit) : 0u;
-    __syncthreads();
-    uint32_t at = s_base + my_emit;
+    if (tid == 0) ncnt[blockIdx.x] = total_emit;
+    size_t at = (size_t)blockIdx.x * SEG_TILE + my_emit;
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const uint32_t slot = tid * PER + k;
@@ -392,6 +434,19 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
             ns[at] = pk[k] - hk[k] + 1;
             at++;
         }
+    }
+}
+
+// The emissions of k_seg_round's tiles (tile t: ecnt[t] entries at t * SEG_TILE) packed after the prefix eoff[t].
+__global__ __launch_bounds__(256) void k_compact_emits(const uint32_t *__restrict__ eh, const uint32_t *__restrict__ es,
+                                                       const uint32_t *__restrict__ ecnt,
+                                                       const uint32_t *__restrict__ eoff, uint32_t *__restrict__ nh,
+                                                       uint32_t *__restrict__ ns) {
+    const uint32_t t = blockIdx.x, c = ecnt[t], o = eoff[t];
+    const size_t src = (size_t)t * SEG_TILE;
+    for (uint32_t k = threadIdx.x; k < c; k += 256) {
+        nh[o + k] = eh[src + k];
+        ns[o + k] = es[src + k];
     }
 }
 
@@ -408,6 +463,10 @@ void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *
     const uint32_t ntiles = (G + 2047) / 2048;
     if (ntiles) k_scan_groups<<<ntiles, 256, 0, s>>>(gs, G, go, tot, tstart, status, counter, epoch, ntiles);
 }
+void launch_compact_emits(const uint32_t *eh, const uint32_t *es, const uint32_t *ecnt, const uint32_t *eoff,
+                          uint32_t ntiles, uint32_t *nh, uint32_t *ns, hipStream_t s) {
+    if (ntiles) k_compact_emits<<<ntiles, 256, 0, s>>>(eh, es, ecnt, eoff, nh, ns);
+}
 void launch_large_elems(const uint32_t *lh, const uint32_t *lo, uint32_t G, uint32_t m, uint32_t *pos, uint32_t *grp,
                         hipStream_t s) {
     uint32_t grid = (m + 255) / 256;
@@ -418,10 +477,10 @@ void launch_seg_round(int mode, const SegRoundArgs &a, hipStream_t s) {
     const uint32_t ntiles = (a.E + SEG_TILE - 1) / SEG_TILE;
     if (!ntiles) return;
     if (mode == 0)
-        k_seg_round<0><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.L, a.T, a.blk, a.blocks, a.n,
+        k_seg_round<0><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.sufl, a.L, a.T, a.blk, a.blocks, a.n,
                                               a.round, a.nh, a.ns, a.ncnt, a.rank_max);
     else
-        k_seg_round<1><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.L, a.T, a.blk, a.blocks, a.n,
+        k_seg_round<1><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.sufl, a.L, a.T, a.blk, a.blocks, a.n,
                                               a.round, a.nh, a.ns, a.ncnt, a.rank_max);
 }
 
