@@ -66,6 +66,31 @@ __device__ __forceinline__ TileRange tile_range(const Tiling &tl, uint32_t t, ui
     return r;
 }
 
+// put(j, T[beg + j]) for every j < cnt, with byte 0 where beg + j >= lim (past the block). The aligned middle of
+// the range moves in 16-byte loads, one or a few per thread and all in flight together; the unaligned head and
+// tail (under 16 bytes each) and the part past lim go byte by byte.
+template <int BLOCK, class F>
+__device__ __forceinline__ void stage_bytes(const uint8_t *__restrict__ T, size_t beg, uint32_t cnt, size_t lim,
+                                            F put) {
+    const int tid = threadIdx.x;
+    const size_t vend = beg + cnt < lim ? beg + cnt : (lim > beg ? lim : beg);   // end of the real bytes
+    const size_t mis = (size_t)(reinterpret_cast<uintptr_t>(T + beg) & 15u);
+    const size_t ab = beg + (mis ? 16 - mis : 0) < vend ? beg + (mis ? 16 - mis : 0) : vend;
+    const size_t ae = ab + ((vend - ab) & ~(size_t)15);
+    const uint4 *vp = reinterpret_cast<const uint4 *>(T + ab);
+    const uint32_t nv = (uint32_t)((ae - ab) >> 4), o = (uint32_t)(ab - beg);
+    for (uint32_t q = tid; q < nv; q += BLOCK) {
+        const uint4 v = vp[q];
+        const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int k = 0; k < 16; k++) put(o + 16 * q + k, (uint8_t)(wd[k >> 2] >> (8 * (k & 3))));
+    }
+    for (uint32_t j = tid; j < o; j += BLOCK) put(j, T[beg + j]);
+    const uint32_t te = (uint32_t)(vend - beg);
+    for (uint32_t j = (uint32_t)(ae - beg) + tid; j < te; j += BLOCK) put(j, T[beg + j]);
+    for (uint32_t j = te + tid; j < cnt; j += BLOCK) put(j, (uint8_t)0);
+}
+
 // ------------------------------------------------------------------ histograms
 //
 // Histogram kernels accumulate in LDS and add their non-zero bins to a global `counts` array, one atomic per bin and
@@ -148,53 +173,60 @@ __device__ __forceinline__ uint64_t build_key(const uint8_t *c, uint8_t bwt_byte
 __global__ __launch_bounds__(256) void k_hist_T(const uint8_t *__restrict__ T, Tiling tl, uint32_t ntiles,
                                                 const uint8_t *__restrict__ codes, uint32_t P,
                                                 uint32_t *__restrict__ counts) {
-    constexpr int HT_COPIES = 4;   // sub-histogram sets, lane l counting into set l % HT_COPIES (see k_byte_hist)
-    __shared__ uint32_t h[HT_COPIES][MAXP][256];
+    // HT_COPIES sub-histogram sets, lane l counting into set l % HT_COPIES (skewed digits collide only within a
+    // set), each bin a 16-bit half of a word: two bins per atomic word keep all sets in 32 KB of LDS. A set sees at
+    // most HT_FLUSH tiles / HT_COPIES rows between flushes, so no half overflows.
+    constexpr int HT_COPIES = 8, HT_FLUSH = 64;
+    static_assert((size_t)HT_FLUSH * HIST_TILE / HT_COPIES < 65536, "16-bit bins");
+    __shared__ uint32_t h[HT_COPIES][MAXP][128];
     __shared__ uint8_t s_code[256];
     __shared__ uint8_t s_c[HIST_TILE + 64];
     const int tid = threadIdx.x, cp = tid % HT_COPIES;
-    for (int i = tid; i < HT_COPIES * MAXP * 256; i += 256) (&h[0][0][0])[i] = 0;
+    for (int i = tid; i < HT_COPIES * MAXP * 128; i += 256) (&h[0][0][0])[i] = 0;
     const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
     const uint32_t t0 = blockIdx.x * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
     int cur = -1;
-    uint32_t w = 0, h0 = 0;
+    uint32_t w = 0, h0 = 0, since = 0;
+    auto flush = [&](uint32_t b) {   // every thread, after a barrier
+        for (uint32_t i = tid; i < P * 256; i += 256) {
+            const uint32_t p = i >> 8, d = i & 255u;
+            uint32_t c = 0;
+#pragma unroll
+            for (int k = 0; k < HT_COPIES; k++) c += (h[k][p][d >> 1] >> (16 * (d & 1))) & 0xFFFFu;
+            if (c) atomicAdd(&counts[(size_t)b * P * 256 + i], c);
+        }
+        __syncthreads();
+        for (int i = tid; i < HT_COPIES * MAXP * 128; i += 256) (&h[0][0][0])[i] = 0;
+    };
     for (uint32_t t = t0; t < t1; t++) {
         TileRange r = tile_range(tl, t, HIST_TILE, 0);
         __syncthreads();
-        if ((int)r.b != cur) {
-            if (cur >= 0) {
-                for (uint32_t i = tid; i < P * 256; i += 256) {
-                    uint32_t c = 0;
-#pragma unroll
-                    for (int k = 0; k < HT_COPIES; k++) { c += (&h[k][0][0])[i]; (&h[k][0][0])[i] = 0; }
-                    if (c) atomicAdd(&counts[(size_t)cur * P * 256 + i], c);
-                }
+        if ((int)r.b != cur || since == (uint32_t)HT_FLUSH) {
+            if (cur >= 0) flush((uint32_t)cur);
+            since = 0;
+            if ((int)r.b != cur) {
+                cur = (int)r.b;
+                s_code[tid] = codes[(size_t)r.b * 256 + tid];
+                w = tl.blocks[r.b].w;
+                h0 = tl.blocks[r.b].h0;
             }
-            cur = (int)r.b;
-            s_code[tid] = codes[(size_t)r.b * 256 + tid];
-            w = tl.blocks[r.b].w;
-            h0 = tl.blocks[r.b].h0;
             __syncthreads();
         }
-        for (uint32_t j = tid; j < HIST_TILE + h0; j += 256) {
-            size_t e = r.beg + j;
-            s_c[j] = e < r.bend ? s_code[T[e]] : 0;
-        }
+        since++;
+        stage_bytes<256>(T, r.beg, HIST_TILE + h0, r.bend,
+                         [&](uint32_t j, uint8_t t) { s_c[j] = r.beg + j < r.bend ? s_code[t] : 0; });
         __syncthreads();
         for (uint32_t j = tid; j < HIST_TILE; j += 256) {
             if (r.beg + j >= r.end) break;
-            uint64_t key = build_key(s_c + j, 0, h0, w);
-            for (uint32_t p = 0; p < P; p++) atomicAdd(&h[cp][p][(uint32_t)(key >> (8 * p + 8)) & 255u], 1u);
+            const uint64_t key = build_key(s_c + j, 0, h0, w);
+            for (uint32_t p = 0; p < P; p++) {
+                const uint32_t d = (uint32_t)(key >> (8 * p + 8)) & 255u;
+                atomicAdd(&h[cp][p][d >> 1], 1u << (16 * (d & 1)));
+            }
         }
     }
     __syncthreads();
-    if (cur >= 0)
-        for (uint32_t i = tid; i < P * 256; i += 256) {
-            uint32_t c = 0;
-#pragma unroll
-            for (int k = 0; k < HT_COPIES; k++) c += (&h[k][0][0])[i];
-            if (c) atomicAdd(&counts[(size_t)cur * P * 256 + i], c);
-        }
+    if (cur >= 0) flush((uint32_t)cur);
 }
 
 // Digit histograms of stored keys at the given shifts (one flat range).
@@ -262,12 +294,10 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
         uint8_t *s_raw = reinterpret_cast<uint8_t *>(s_buf);
         uint8_t *s_cod = s_raw + TILE + 64;
         __syncthreads();
-        for (uint32_t j = tid; j < (uint32_t)TILE + h0; j += BLOCK) {
-            size_t e = base + j;
-            uint8_t t = e < tr.bend ? T[e] : 0;
-            s_cod[j] = e < tr.bend ? s_code[t] : 0;
+        stage_bytes<BLOCK>(T, base, (uint32_t)TILE + h0, tr.bend, [&](uint32_t j, uint8_t t) {
+            s_cod[j] = base + j < tr.bend ? s_code[t] : 0;
             if (j + 1 <= (uint32_t)TILE) s_raw[j + 1] = t;
-        }
+        });
         if (tid == 0) s_raw[0] = base > boff ? T[base - 1] : 0;
         __syncthreads();
 #pragma unroll
