@@ -295,8 +295,10 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
                 const uint32_t i = s_i[e];
                 const uint32_t head = s_gh[lg] + less, pnew = head + eqb;
                 if (eq != en - st) {             // the group splits (else SA, ISA stay as they are)
-                    SA[pnew] = i;
-                    ISA[i] = head;
+                    // only what changes: a row that keeps its SA position or its head (the first subgroup keeps
+                    // the group's) writes nothing there
+                    if (pnew != s_gh[lg] + (e - st)) SA[pnew] = i;
+                    if (less) ISA[i] = head;
                     if (eq == 1) L[pnew] = i ? T[i - 1] : (uint8_t)0;   // a row's byte is written once it is alone
                 }
                 if (eq >= 2 && eqb == eq - 1) { eh[k] = head; es[k] = eq; nemit++; }
@@ -368,7 +370,7 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     // position of its first slot (max-scan over slots: the groups' heads themselves come in any order), and each
     // kept subgroup's last slot emits it into the next list
     constexpr int PER = CAP / 256;
-    uint32_t pk[PER], hk[PER], ik[PER], gk[PER];
+    uint32_t pk[PER], hk[PER], ik[PER], gk[PER], ek[PER];
     bool flag[PER], lastk[PER];
     uint32_t run_max = 0, nemit = 0;
     const uint64_t *buf = s_buf + src * CAP;
@@ -380,12 +382,14 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         pk[k] = 0;
         ik[k] = 0;
         gk[k] = 0;
+        ek[k] = 0;
         if (slot < cnt) {
             const uint64_t key = buf[slot] >> 12;
             const uint32_t lg = (uint32_t)(key >> 32);
             gk[k] = lg;
             pk[k] = s_gh[lg] + (slot - s_go[lg]);
-            ik[k] = s_i[buf[slot] & 0xFFFu];
+            ek[k] = (uint32_t)(buf[slot] & 0xFFFu);   // the row's slot before the sort
+            ik[k] = s_i[ek[k]];
             flag[k] = slot == 0 || (buf[slot - 1] >> 12) != key;
             lastk[k] = slot + 1 == cnt || (buf[slot + 1] >> 12) != key;
         }
@@ -414,9 +418,9 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
             const uint32_t hslot = hk[k] > before_me ? hk[k] : before_me;
             const uint32_t head = s_gh[gk[k]] + (hslot - s_go[gk[k]]);
             hk[k] = head;
-            if (s_nsub[gk[k]] > 1) {
-                SA[pk[k]] = ik[k];
-                ISA[ik[k]] = head;
+            if (s_nsub[gk[k]] > 1) {   // only what changes (see the direct ranking above)
+                if (pk[k] != s_gh[gk[k]] + (ek[k] - s_go[gk[k]])) SA[pk[k]] = ik[k];
+                if (head != s_gh[gk[k]]) ISA[ik[k]] = head;
                 if (flag[k] && lastk[k]) L[pk[k]] = ik[k] ? T[ik[k] - 1] : (uint8_t)0;
             }
             if (lastk[k] && !flag[k]) nemit++;
