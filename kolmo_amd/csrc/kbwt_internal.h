@@ -16,7 +16,8 @@ constexpr int HIST_TILE = 4096;              // bytes per tile of the histogram 
 constexpr int HIST_GRID = 512;               // workgroups of the histogram kernels
 constexpr int WALK_TILE = 256;               // splitters per workgroup of the inverse walks
 constexpr uint32_t MIN_TILE = 2048;          // smallest tile of any tiled kernel (sizes the lookback arrays)
-constexpr int SEG_TILE = 1024;               // doubling rounds: groups of up to SEG_TILE rows sort inside one workgroup
+constexpr int SEG_TILE = 1024;               // doubling rounds: rows of the group list per tile (by first row)
+constexpr int SEG_GROUP = 512;               // groups of up to SEG_GROUP rows sort inside one workgroup
 constexpr int SEG_RANK_MAX = 16;             // default: tiles whose groups all have at most this many rows rank directly
 
 // One block of a batch as the kernels see it.

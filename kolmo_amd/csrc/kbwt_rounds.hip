@@ -29,7 +29,7 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *go, uint32_t
 
 // Element rows of compacted active rows (pos_out, grp_out: SA position and group head of each, in SA order, groups
 // contiguous) to group descriptors: the last row of each group emits (head, size = pos - head + 1) into the small
-// list (size <= SEG_TILE) or the large list. A workgroup takes tiles of E2G_TILE rows and makes one atomic append per
+// list (size <= SEG_GROUP) or the large list. A workgroup takes tiles of E2G_TILE rows and makes one atomic append per
 // list and tile (appends to one counter from every wave serialise at the memory side); the lists' order is free.
 // *mp: rows.
 constexpr int E2G_ITEMS = 16, E2G_TILE = 256 * E2G_ITEMS;
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(256) void k_elems_to_groups(const uint32_t *__restr
             const bool last = o < m && (o + 1 == m || nh != head[k]);
             if (last) {
                 kind |= 1u << k;
-                if (size[k] > (uint32_t)SEG_TILE) { kind |= 1u << (16 + k); nl++; } else ns++;
+                if (size[k] > (uint32_t)SEG_GROUP) { kind |= 1u << (16 + k); nl++; } else ns++;
             }
         }
         uint32_t tot;
@@ -171,8 +171,10 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
                                                    const BlockInfo *__restrict__ blocks, uint32_t n, uint32_t round,
                                                    uint32_t *__restrict__ nh, uint32_t *__restrict__ ns,
                                                    uint32_t *__restrict__ ncnt, uint32_t rank_max) {
-    constexpr int CAP = 2 * SEG_TILE;          // a tile's rows: groups starting in SEG_TILE, each <= SEG_TILE rows
-    constexpr int MAXG = SEG_TILE / 2 + 1;     // groups starting in the tile (two rows or more each)
+    // a tile's rows: the groups starting in its SEG_TILE rows, each of at most SEG_GROUP rows. CAP keeps the LDS of
+    // a workgroup within a quarter of a CU's, so four workgroups share a CU (the rounds are latency-bound)
+    constexpr int CAP = SEG_TILE + SEG_GROUP;
+    constexpr int MAXG = SEG_TILE / 2;         // groups starting in the tile (two rows or more each)
     constexpr int ITEMS = CAP / 256;
     constexpr int W = 4;
     __shared__ uint32_t s_gh[MAXG], s_go[MAXG];
@@ -397,7 +399,8 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         run_max = run_max > cand ? run_max : cand;
         hk[k] = run_max;   // within-thread running max of subgroup-start slots, fixed below with earlier threads
     }
-    __shared__ uint32_t s_nsub[MAXG];
+    uint32_t *const s_nsub = &s_wcnt[0][0];   // the radix counters are free after the last pass
+    static_assert(W * 256 >= MAXG, "subgroup counts in the counters' LDS");
     for (uint32_t j = tid; j < ng; j += 256) s_nsub[j] = 0;
     __syncthreads();
 #pragma unroll
