@@ -230,7 +230,7 @@ Tuning tuning_for(uint32_t N) {
     Tuning t = g_tune[size_class(N)];
     t.sort_cfg = env_u32("KBWT_SORT_CFG", t.sort_cfg, 0, SORT_NCFG - 1);
     t.inv_dshift = env_u32("KBWT_INV_DSHIFT", t.inv_dshift, 2, 12);
-    t.seg_rank = env_u32("KBWT_SEG_RANK", t.seg_rank, 0, SEG_TILE);
+    t.seg_rank = env_u32("KBWT_SEG_RANK", t.seg_rank, 0, 1024);
     return t;
 }
 
@@ -1055,7 +1055,7 @@ int32_t kbwt_dev_unbwt_aux_batch(const uint8_t *dT, uint8_t *dU, const kbwt_seg 
 
 static bool tuning_ok(const kbwt_tuning *t) {
     return t && t->sort_cfg >= 0 && t->sort_cfg < SORT_NCFG && t->inv_dshift >= 2 && t->inv_dshift <= 12 &&
-           t->seg_rank >= 0 && t->seg_rank <= SEG_TILE;
+           t->seg_rank >= 0 && t->seg_rank <= 1024;
 }
 
 int32_t kbwt_set_tuning(int32_t size_log2, const kbwt_tuning *t) {
