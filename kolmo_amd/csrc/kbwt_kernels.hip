@@ -675,6 +675,7 @@ __global__ __launch_bounds__(BLOCK) void k_lf_pass(const uint8_t *__restrict__ L
     constexpr int TILE = BLOCK * ITEMS;
     __shared__ uint32_t s_wcnt[W][256];
     __shared__ uint32_t s_base[256];
+    __shared__ uint8_t s_l[TILE];          // the tile's bytes, staged with 16-byte loads
     __shared__ uint32_t s_tile;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
     if (tid == 0) s_tile = next_tile(counter, ntiles);
@@ -683,12 +684,14 @@ __global__ __launch_bounds__(BLOCK) void k_lf_pass(const uint8_t *__restrict__ L
     const uint32_t tile = s_tile;
     const TileRange tr = tile_range(tl, tile, TILE, 0);
     const uint32_t cnt = (uint32_t)(tr.end - tr.beg);
+    stage_bytes<BLOCK>(L, tr.beg, cnt, tr.end, [&](uint32_t j, uint8_t c) { s_l[j] = c; });
+    __syncthreads();
     uint32_t dg[ITEMS], rk[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
         bool valid = local < cnt;
-        uint32_t d = valid ? L[tr.beg + local] : 0u;
+        uint32_t d = valid ? s_l[local] : 0u;
         uint64_t peers = match_digit8(d, __ballot(valid));
         dg[k] = d;
         rk[k] = 0;
