@@ -54,11 +54,16 @@ struct Workspace {
     uint32_t epoch = 1;
     Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, st_max, samples, stage, blk, counts, dbase,
         tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es, ecnt, eoff;
-    uint32_t *counters = nullptr;           // 16 tile counters (zeroed once)
+    uint32_t *counters = nullptr;           // 16 tile counters (zeroed once); [15]: the device epoch of graphs
     uint32_t *gcnt = nullptr;               // 4 group-list counters
     uint32_t *h_gcnt = nullptr;
     unsigned long long *orand = nullptr;    // 2
     uint32_t *m_out = nullptr;              // 1
+    // the inverse's launch sequence as an executable graph, for a call that repeats the previous call's shape
+    struct {
+        std::vector<uint64_t> key, last;
+        hipGraphExec_t exec = nullptr;
+    } inv_graph;
     // pinned host staging
     void *h_meta = nullptr;
     size_t h_meta_cap = 0;
@@ -144,11 +149,57 @@ Workspace &workspace() {
         HIPCHK(hipHostMalloc(&w.h_m, sizeof(uint32_t), hipHostMallocDefault));
         w.h_orand_init[0] = 0ull;
         w.h_orand_init[1] = ~0ull;
-        HIPCHK(hipEventCreate(&w.ev0));
+        {   // the device epoch starts in the upper half of the epoch space
+        const uint32_t e0 = EPOCH_HALF;
+        HIPCHK(hipMemcpy(w.counters + 15, &e0, 4, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipEventCreate(&w.ev0));
         HIPCHK(hipEventCreate(&w.ev1));
         w.ready = true;
     }
     return w;
+}
+
+uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi);
+
+// Host-side lookback epochs stay in [1, EPOCH_HALF); graphs use the upper half.
+inline uint32_t next_epoch(Workspace &w) {
+    const uint32_t e = w.epoch;
+    w.epoch = w.epoch + 1 < EPOCH_HALF ? w.epoch + 1 : 1u;
+    return e;
+}
+
+bool prof_enabled() {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    return g_prof_on;
+}
+
+// Run `body` (a fixed launch sequence on stream s) directly, or from an executable graph: a call whose key (shapes
+// and buffers) equals the previous call's captures the sequence once and then replays it. Graphs are skipped with
+// per-kernel profiling on (events bracket every launch), on the legacy default stream (it cannot be captured) and
+// with KBWT_GRAPHS=0.
+template <class G, class F> void run_graph(G &g, const std::vector<uint64_t> &key, hipStream_t s, F body) {
+    const bool eligible = s != nullptr && !prof_enabled() && env_u32("KBWT_GRAPHS", 1, 0, 1);
+    if (!eligible || key != g.last) {
+        g.last = key;
+        body();
+        return;
+    }
+    if (!g.exec || g.key != key) {
+        if (g.exec) {
+            (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        body();
+        HIPCHK(hipStreamEndCapture(s, &graph));
+        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHK(e);
+        g.key = key;
+    }
+    HIPCHK(hipGraphLaunch(g.exec, s));
 }
 
 // Launch `fn` on stream s; with profiling on, bracket it with a pair of events.
@@ -405,7 +456,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         a.tiling = tsort;
         a.status = P<uint64_t>(w.status);
         a.counter = w.counters + 0;
-        a.epoch = w.epoch++;
+        a.epoch = next_epoch(w);
         a.T = dT;
         a.codes = d_codes;
         TIMED(p == 0 ? "sort_pass_T" : "sort_pass", N, launch_sort_pass(a, nst, s));
@@ -434,7 +485,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         g.st_sum = P<uint64_t>(w.st_sum);
         g.st_max = P<uint64_t>(w.st_max);
         g.counter = w.counters + 1;
-        g.epoch = w.epoch++;
+        g.epoch = next_epoch(w);
         TIMED("regroup", N, launch_regroup(g, s));
         LAUNCHCHK();
         // ISA[suffix] = head: partition the pairs by suffix window (one keys-only radix pass), then write window by
@@ -442,7 +493,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         const uint32_t wshift = isa_window_shift(N);
         TIMED("isa_part", N,
               launch_partition_pairs(K[P0 & 1], K[(P0 - 1) & 1], N, wshift, P<uint64_t>(w.status), w.counters + 4,
-                                     w.epoch++, s));
+                                     next_epoch(w), s));
         LAUNCHCHK();
         TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, wshift, ISA, s));
     }
@@ -511,7 +562,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         if (Gs) {
             TIMED("scan_groups", Gs,
                   launch_scan_groups(P<uint32_t>(w.gs[cur]), Gs, P<uint32_t>(w.go), w.m_out, P<uint32_t>(w.tstart),
-                                     P<uint64_t>(w.st_sum), w.counters + 3, w.epoch++, s));
+                                     P<uint64_t>(w.st_sum), w.counters + 3, next_epoch(w), s));
             LAUNCHCHK();
             HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
@@ -574,7 +625,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
             const uint32_t nt = (Es + SEG_TILE - 1) / SEG_TILE;
             TIMED("emit_compact", Es,
                   launch_scan_groups(P<uint32_t>(w.ecnt), nt, P<uint32_t>(w.eoff), gcnt, nullptr,
-                                     P<uint64_t>(w.st_sum), w.counters + 3, w.epoch++, s);
+                                     P<uint64_t>(w.st_sum), w.counters + 3, next_epoch(w), s);
                   launch_compact_emits(P<uint32_t>(w.eh), P<uint32_t>(w.es), P<uint32_t>(w.ecnt), P<uint32_t>(w.eoff),
                                        nt, P<uint32_t>(w.gh[cur ^ 1]), P<uint32_t>(w.gs[cur ^ 1]), s));
         }
@@ -603,7 +654,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
                     a.tiling = flat;
                     a.status = P<uint64_t>(w.status);
                     a.counter = w.counters + 0;
-                    a.epoch = w.epoch++;
+                    a.epoch = next_epoch(w);
                     TIMED("sort_pass", ml, launch_sort_pass(a, ntl, s));
                     LAUNCHCHK();
                     src ^= 1;
@@ -626,7 +677,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
             g.st_sum = P<uint64_t>(w.st_sum);
             g.st_max = P<uint64_t>(w.st_max);
             g.counter = w.counters + 1;
-            g.epoch = w.epoch++;
+            g.epoch = next_epoch(w);
             TIMED("regroup", ml, launch_regroup(g, s));
             TIMED("to_groups", ml,
                   launch_elems_to_groups(P<uint32_t>(w.pos[1]), P<uint32_t>(w.grp[1]), w.m_out, ml,
@@ -715,33 +766,41 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     const uint16_t *blk = B > 1 ? P<uint16_t>(w.blk) : nullptr;
 
     HIPCHK(hipEventRecord(w.ev0, s));
-    launch_fill_u8(P<uint8_t>(w.counts), 0, (size_t)B * MAXP * 256 * 4, s);
-    if (B > 1) TIMED("fill_blk", N, launch_fill_blk(P<uint16_t>(w.blk), d_bl, B, N, s));
-    TIMED("byte_hist", N, launch_byte_hist(dL, thist, nht, P<uint32_t>(w.counts), s));
-    TIMED("scan_digits", 256 * B,
-          launch_scan_digits(P<uint32_t>(w.counts), B, 1, d_bl, 2, nullptr, P<uint32_t>(w.cbase), s));
     uint32_t *lf = P<uint32_t>(w.V0);
-    TIMED("lf_pass", N,
-          launch_lf_pass(dL, tlf, nlt, P<uint32_t>(w.cbase), lf, P<uint64_t>(w.status), w.counters + 2, w.epoch++, s));
-    LAUNCHCHK();
-
     uint64_t *link[2] = {P<uint64_t>(w.K0), P<uint64_t>(w.K1)};
     uint32_t *len = P<uint32_t>(w.pos[0]), *cont = P<uint32_t>(w.pos[1]);
     uint8_t *stage = P<uint8_t>(w.stage);
-    TIMED("walk_count", N,
-          launch_walk_count(lf, twalk, nwt, dshift, S, P<uint32_t>(w.cbase), link[0], len, stage, sstride, cap, cont,
-                            s));
-    LAUNCHCHK();
     const uint32_t iters = rank_jump_iters(S);
-    int cur = 0;
-    for (uint32_t it = 0; it < iters; it++) {
-        TIMED("rank_jump", S, launch_rank_jump(link[cur], link[cur ^ 1], S, s));
-        cur ^= 1;
-    }
-    launch_check_lists(link[cur], d_bl, B, S, P<uint32_t>(w.bad), s);
-    TIMED("walk_emit", N,
-          launch_walk_emit(lf, twalk, nwt, dshift, S, link[cur], len, P<uint32_t>(w.cbase), stage, sstride, cap,
-                           cont, P<uint32_t>(w.bad), dOut, s));
+    uint32_t *depoch = w.counters + 15;
+    // the launch sequence depends only on these shapes and buffers (the data travel in the metadata buffer)
+    const std::vector<uint64_t> key = {
+        B, N, S, dshift, cap, sstride, nht, nlt, nwt, (uintptr_t)dL, (uintptr_t)dOut, (uintptr_t)dm,
+        (uintptr_t)w.V0.p, (uintptr_t)w.K0.p, (uintptr_t)w.K1.p, (uintptr_t)w.pos[0].p, (uintptr_t)w.pos[1].p,
+        (uintptr_t)w.stage.p, (uintptr_t)w.status.p, (uintptr_t)w.counts.p, (uintptr_t)w.cbase.p,
+        (uintptr_t)w.bad.p, (uintptr_t)w.blk.p, (uintptr_t)w.counters};
+    run_graph(w.inv_graph, key, s, [&] {
+        launch_epoch_bump(depoch, s);
+        launch_fill_u8(P<uint8_t>(w.counts), 0, (size_t)B * MAXP * 256 * 4, s);
+        if (B > 1) TIMED("fill_blk", N, launch_fill_blk(P<uint16_t>(w.blk), d_bl, B, N, s));
+        TIMED("byte_hist", N, launch_byte_hist(dL, thist, nht, P<uint32_t>(w.counts), s));
+        TIMED("scan_digits", 256 * B,
+              launch_scan_digits(P<uint32_t>(w.counts), B, 1, d_bl, 2, nullptr, P<uint32_t>(w.cbase), s));
+        TIMED("lf_pass", N,
+              launch_lf_pass(dL, tlf, nlt, P<uint32_t>(w.cbase), lf, P<uint64_t>(w.status), w.counters + 2, 0u, depoch,
+                             s));
+        TIMED("walk_count", N,
+              launch_walk_count(lf, twalk, nwt, dshift, S, P<uint32_t>(w.cbase), link[0], len, stage, sstride, cap,
+                                cont, s));
+        int cur = 0;
+        for (uint32_t it = 0; it < iters; it++) {
+            TIMED("rank_jump", S, launch_rank_jump(link[cur], link[cur ^ 1], S, s));
+            cur ^= 1;
+        }
+        launch_check_lists(link[cur], d_bl, B, S, P<uint32_t>(w.bad), s);
+        TIMED("walk_emit", N,
+              launch_walk_emit(lf, twalk, nwt, dshift, S, link[cur], len, P<uint32_t>(w.cbase), stage, sstride, cap,
+                               cont, P<uint32_t>(w.bad), dOut, s));
+    });
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(h_bad, w.bad.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -1284,6 +1343,12 @@ void kbwt_release(void) {
                    &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
                    &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
                    &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,  &w.sufl,     &w.eh,     &w.es,     &w.ecnt,   &w.eoff};
+    if (w.inv_graph.exec) {
+        (void)hipGraphExecDestroy(w.inv_graph.exec);
+        w.inv_graph.exec = nullptr;
+    }
+    w.inv_graph.key.clear();
+    w.inv_graph.last.clear();
     for (Buf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;

@@ -139,7 +139,11 @@ void launch_samples(const uint32_t *ISA, const BlockInfo *blocks, uint32_t B, ui
                     hipStream_t s);
 void launch_fill_blk(uint16_t *blk, const BlockInfo *blocks, uint32_t B, uint32_t N, hipStream_t s);
 void launch_lf_pass(const uint8_t *L, Tiling tl, uint32_t ntiles, const uint32_t *cbase, uint32_t *lf,
-                    uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s);
+                    uint64_t *status, uint32_t *counter, uint32_t epoch, const uint32_t *depoch, hipStream_t s);
+// Lookback epochs: the host numbers launches in [1, EPOCH_HALF); a captured graph reads a device counter in
+// [EPOCH_HALF, 2 * EPOCH_HALF) that a bump kernel at its start moves on by EPOCH_STRIDE.
+constexpr uint32_t EPOCH_HALF = 1u << 29, EPOCH_STRIDE = 64;
+void launch_epoch_bump(uint32_t *depoch, hipStream_t s);
 void launch_walk_count(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
                        const uint32_t *cbase, uint64_t *link, uint32_t *len, uint8_t *stage, size_t sstride,
                        uint32_t cap, uint32_t *cont, hipStream_t s);

@@ -670,7 +670,9 @@ template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_lf_pass(const uint8_t *__restrict__ L, Tiling tl,
                                                    const uint32_t *__restrict__ cbase, uint32_t *__restrict__ lf,
                                                    uint64_t *__restrict__ status, uint32_t *__restrict__ counter,
-                                                   uint32_t epoch, uint32_t ntiles) {
+                                                   uint32_t epoch_arg, const uint32_t *__restrict__ depoch,
+                                                   uint32_t ntiles) {
+    const uint32_t epoch = depoch ? *depoch + epoch_arg : epoch_arg;   // a replayed graph reads its epoch here
     constexpr int W = BLOCK / WAVE;
     constexpr int TILE = BLOCK * ITEMS;
     __shared__ uint32_t s_wcnt[W][256];
@@ -1002,10 +1004,18 @@ void launch_fill_blk(uint16_t *blk, const BlockInfo *blocks, uint32_t B, uint32_
     if (B) k_fill_blk<<<dim3(64, B), 256, 0, s>>>(blk, blocks);
 }
 void launch_lf_pass(const uint8_t *L, Tiling tl, uint32_t ntiles, const uint32_t *cbase, uint32_t *lf,
-                    uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s) {
+                    uint64_t *status, uint32_t *counter, uint32_t epoch, const uint32_t *depoch, hipStream_t s) {
     if (ntiles)
-        k_lf_pass<LF_BLOCK, LF_ITEMS><<<ntiles, LF_BLOCK, 0, s>>>(L, tl, cbase, lf, status, counter, epoch, ntiles);
+        k_lf_pass<LF_BLOCK, LF_ITEMS><<<ntiles, LF_BLOCK, 0, s>>>(L, tl, cbase, lf, status, counter, epoch, depoch,
+                                                                 ntiles);
 }
+// Device-side epochs of a replayed graph: *depoch moves on by EPOCH_STRIDE per replay, within the upper half of
+// the epoch space (host epochs stay in the lower half).
+__global__ void k_epoch_bump(uint32_t *depoch) {
+    const uint32_t v = *depoch + EPOCH_STRIDE;
+    *depoch = (v & (EPOCH_HALF - 1)) | EPOCH_HALF;
+}
+void launch_epoch_bump(uint32_t *depoch, hipStream_t s) { k_epoch_bump<<<1, 1, 0, s>>>(depoch); }
 void launch_walk_count(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
                        const uint32_t *cbase, uint64_t *link, uint32_t *len, uint8_t *stage, size_t sstride,
                        uint32_t cap, uint32_t *cont, hipStream_t s) {

@@ -166,3 +166,21 @@ def test_gpu_round_paths(gpu, rank_max, monkeypatch):
         T = K.generate(kind, 31, n)
         say("round paths", rank_max, kind, n)
         check_case(T, 1 << K.codec18_rlog(n), f"{kind} n {n} rank_max {rank_max}")
+
+
+def test_gpu_inverse_graph_replays(gpu):
+    """Repeated inverses of one shape run from a captured HIP graph (the second call with a shape captures it, later
+    calls replay it); every replay decodes its own input, also with forward sorts and other shapes in between that
+    reuse the lookback status words and the workspace."""
+    n = 3 << 19
+    r = 1 << K.codec18_rlog(n)
+    for i, kind in enumerate(["mixed", "dna", "repeats", "logs", "mixed", "csvcols"]):
+        T = K.generate(kind, 40 + i, n)
+        say("graph replay", i, kind, n)
+        U, I = K.bwt_aux(T, r)
+        assert np.array_equal(K.unbwt_aux(U, r, I), T), f"replay {i} {kind}"
+        if i == 3:                                   # another shape, then back to the captured one
+            T2 = K.generate("dna", 7, 100000)
+            r2 = 1 << K.codec18_rlog(T2.size)
+            U2, I2 = K.bwt_aux(T2, r2)
+            assert np.array_equal(K.unbwt_aux(U2, r2, I2), T2)
