@@ -145,7 +145,8 @@ KBWT_API int32_t kbwt_last_stats(kbwt_stats *out);
 /* ---- measurement (no counterpart in the reference) ----
    kbwt_profile(1) clears and starts per-kernel timing: every launch is bracketed by HIP events on its own stream, and
    kbwt_profile_read fills up to max records (name, summed device ms, launches, summed elements) and returns their
-   count. kbwt_profile(0) stops it. kbwt_dev_copy is a plain 16-byte-vector device copy (16-byte aligned pointers and
+   count. kbwt_profile(0) stops it. kbwt_profile_only(name) clears and times only the launches of that kernel, so
+   a timed run carries events around those launches alone. kbwt_dev_copy is a plain 16-byte-vector device copy (16-byte aligned pointers and
    size), the reference workload for scaling hardware byte counters. */
 typedef struct {
     char name[32];
@@ -154,6 +155,7 @@ typedef struct {
     int64_t elems;
 } kbwt_kernel_time;
 KBWT_API void kbwt_profile(int32_t on);
+KBWT_API void kbwt_profile_only(const char *name);
 KBWT_API int32_t kbwt_profile_read(kbwt_kernel_time *out, int32_t max);
 KBWT_API int32_t kbwt_dev_copy(void *dst, const void *src, size_t bytes, void *stream);
 

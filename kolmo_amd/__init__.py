@@ -94,6 +94,8 @@ def lib():
         L.kbwt_release.restype = None
         L.kbwt_profile.restype = None
         L.kbwt_profile.argtypes = [ctypes.c_int32]
+        L.kbwt_profile_only.restype = None
+        L.kbwt_profile_only.argtypes = [ctypes.c_char_p]
         L.kbwt_profile_read.restype = ctypes.c_int32
         L.kbwt_profile_read.argtypes = [ctypes.POINTER(KernelTime), ctypes.c_int32]
         for name in ("kbwt_bwt_aux_batch", "kbwt_unbwt_aux_batch"):

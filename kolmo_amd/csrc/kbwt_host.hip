@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <new>
 #include <string>
@@ -53,7 +54,7 @@ struct Workspace {
     std::mutex mu;
     uint32_t epoch = 1;
     Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, st_max, samples, stage, blk, counts, dbase,
-        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es, ecnt, eoff;
+        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es, ecnt, eoff, pair;
     uint32_t *counters = nullptr;           // 16 tile counters (zeroed once); [15]: the device epoch of graphs
     uint32_t *gcnt = nullptr;               // 4 group-list counters
     uint32_t *h_gcnt = nullptr;
@@ -89,7 +90,10 @@ struct KernelTime {
 };
 std::mutex g_prof_mu;
 bool g_prof_on = false;
+std::string g_prof_only;   // when not empty, only launches of this name are timed
 std::vector<KernelTime> g_prof;
+
+bool prof_wants(const char *name) { return g_prof_on && (g_prof_only.empty() || g_prof_only == name); }
 
 int prof_slot(const char *name) {
     for (size_t i = 0; i < g_prof.size(); i++)
@@ -169,17 +173,22 @@ inline uint32_t next_epoch(Workspace &w) {
     return e;
 }
 
-bool prof_enabled() {
+// Whether any of the named launches would be bracketed by profiling events.
+bool prof_touches(std::initializer_list<const char *> names) {
     std::lock_guard<std::mutex> lk(g_prof_mu);
-    return g_prof_on;
+    for (const char *n : names)
+        if (prof_wants(n)) return true;
+    return false;
 }
 
-// Run `body` (a fixed launch sequence on stream s) directly, or from an executable graph: a call whose key (shapes
-// and buffers) equals the previous call's captures the sequence once and then replays it. Graphs are skipped with
-// per-kernel profiling on (events bracket every launch), on the legacy default stream (it cannot be captured) and
-// with KBWT_GRAPHS=0.
-template <class G, class F> void run_graph(G &g, const std::vector<uint64_t> &key, hipStream_t s, F body) {
-    const bool eligible = s != nullptr && !prof_enabled() && env_u32("KBWT_GRAPHS", 1, 0, 1);
+// Run `body` (a fixed launch sequence on stream s, whose launches are `names`) directly, or from an executable
+// graph: a call whose key (shapes and buffers) equals the previous call's captures the sequence once and then
+// replays it. Graphs are skipped while profiling times one of the launches (events bracket them), on the legacy
+// default stream (it cannot be captured) and with KBWT_GRAPHS=0.
+template <class G, class F>
+void run_graph(G &g, const std::vector<uint64_t> &key, hipStream_t s, std::initializer_list<const char *> names,
+               F body) {
+    const bool eligible = s != nullptr && !prof_touches(names) && env_u32("KBWT_GRAPHS", 1, 0, 1);
     if (!eligible || key != g.last) {
         g.last = key;
         body();
@@ -208,7 +217,7 @@ template <class F> void timed(Workspace &w, hipStream_t s, const char *name, uin
     int slot = 0;
     {
         std::lock_guard<std::mutex> lk(g_prof_mu);
-        on = g_prof_on;
+        on = prof_wants(name);
         if (on) slot = prof_slot(name);
     }
     if (!on) {
@@ -736,7 +745,9 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     grow(w.K1, ((size_t)S + 1) * 8);           // links (pong)
     // step-major stage: cap rows of sstride bytes, at most STAGE_MAX bytes in all
     const size_t sstride = ((size_t)S + 63) & ~(size_t)63;
-    if ((size_t)cap * sstride > STAGE_MAX) cap = (uint32_t)std::max<size_t>(1, STAGE_MAX / sstride);
+    if ((size_t)cap * sstride > STAGE_MAX) cap = (uint32_t)std::max<size_t>(2, STAGE_MAX / sstride);
+    cap = std::max(2u, cap & ~1u);             // the counting walk stages two steps at a time
+    grow(w.pair, NN * 8);                      // two LF steps per row
     grow(w.stage, (size_t)cap * sstride);
     const size_t ntiles_max = (NN + LF_TILE - 1) / LF_TILE + B;
     grow(w.status, ntiles_max * 256 * 8, true);
@@ -777,8 +788,9 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
         B, N, S, dshift, cap, sstride, nht, nlt, nwt, (uintptr_t)dL, (uintptr_t)dOut, (uintptr_t)dm,
         (uintptr_t)w.V0.p, (uintptr_t)w.K0.p, (uintptr_t)w.K1.p, (uintptr_t)w.pos[0].p, (uintptr_t)w.pos[1].p,
         (uintptr_t)w.stage.p, (uintptr_t)w.status.p, (uintptr_t)w.counts.p, (uintptr_t)w.cbase.p,
-        (uintptr_t)w.bad.p, (uintptr_t)w.blk.p, (uintptr_t)w.counters};
-    run_graph(w.inv_graph, key, s, [&] {
+        (uintptr_t)w.bad.p, (uintptr_t)w.blk.p, (uintptr_t)w.counters, (uintptr_t)w.pair.p};
+    run_graph(w.inv_graph, key, s,
+              {"fill_blk", "byte_hist", "scan_digits", "lf_pass", "lf_pairs", "walk_count", "rank_jump", "walk_emit"}, [&] {
         launch_epoch_bump(depoch, s);
         launch_fill_u8(P<uint8_t>(w.counts), 0, (size_t)B * MAXP * 256 * 4, s);
         if (B > 1) TIMED("fill_blk", N, launch_fill_blk(P<uint16_t>(w.blk), d_bl, B, N, s));
@@ -788,9 +800,10 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
         TIMED("lf_pass", N,
               launch_lf_pass(dL, tlf, nlt, P<uint32_t>(w.cbase), lf, P<uint64_t>(w.status), w.counters + 2, 0u, depoch,
                              s));
+        TIMED("lf_pairs", N, launch_lf_pairs(lf, dL, tlf, nlt, dshift, P<uint64_t>(w.pair), s));
         TIMED("walk_count", N,
-              launch_walk_count(lf, twalk, nwt, dshift, S, P<uint32_t>(w.cbase), link[0], len, stage, sstride, cap,
-                                cont, s));
+              launch_walk_count(P<uint64_t>(w.pair), twalk, nwt, dshift, S, P<uint32_t>(w.cbase), link[0], len, stage,
+                                sstride, cap, cont, s));
         int cur = 0;
         for (uint32_t it = 0; it < iters; it++) {
             TIMED("rank_jump", S, launch_rank_jump(link[cur], link[cur ^ 1], S, s));
@@ -1286,6 +1299,14 @@ int32_t kbwt_unbwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs,
 void kbwt_profile(int32_t on) {
     std::lock_guard<std::mutex> lk(g_prof_mu);
     g_prof_on = on != 0;
+    g_prof_only.clear();
+    g_prof.clear();
+}
+
+void kbwt_profile_only(const char *name) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_on = name != nullptr && name[0] != 0;
+    g_prof_only = g_prof_on ? name : "";
     g_prof.clear();
 }
 
@@ -1342,7 +1363,7 @@ void kbwt_release(void) {
                    &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.st_max, &w.samples,
                    &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
                    &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
-                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,  &w.sufl,     &w.eh,     &w.es,     &w.ecnt,   &w.eoff};
+                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,  &w.sufl,     &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair};
     if (w.inv_graph.exec) {
         (void)hipGraphExecDestroy(w.inv_graph.exec);
         w.inv_graph.exec = nullptr;

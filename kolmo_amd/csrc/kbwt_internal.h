@@ -144,7 +144,9 @@ void launch_lf_pass(const uint8_t *L, Tiling tl, uint32_t ntiles, const uint32_t
 // [EPOCH_HALF, 2 * EPOCH_HALF) that a bump kernel at its start moves on by EPOCH_STRIDE.
 constexpr uint32_t EPOCH_HALF = 1u << 29, EPOCH_STRIDE = 64;
 void launch_epoch_bump(uint32_t *depoch, hipStream_t s);
-void launch_walk_count(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
+void launch_lf_pairs(const uint32_t *lf, const uint8_t *L, Tiling tl, uint32_t ntiles, uint32_t dshift,
+                     uint64_t *pair, hipStream_t s);
+void launch_walk_count(const uint64_t *pair, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
                        const uint32_t *cbase, uint64_t *link, uint32_t *len, uint8_t *stage, size_t sstride,
                        uint32_t cap, uint32_t *cont, hipStream_t s);
 uint32_t rank_jump_iters(uint32_t S);

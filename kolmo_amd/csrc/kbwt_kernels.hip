@@ -735,16 +735,48 @@ __device__ __forceinline__ uint8_t row_byte(const uint32_t *s_c, uint32_t v) {
     return (uint8_t)lo;
 }
 
+// Two LF steps per load for the counting walk. For the stored row q of block b (row r = q - off, one more from P on):
+// pair[q] = (stop << 40) | (L[q] << 32) | x, where L[q] is the byte of the step from r and LF(r) its landing row.
+// stop: LF(r) is a splitter row (row % D == 0, or P), and then x = LF(r), where the walk ends; else x = LF(LF(r)).
+// The second lookup follows LF, which is increasing over the rows of one byte, so the loads of a tile fall into one
+// short run per byte value.
+__global__ __launch_bounds__(LF_BLOCK) void k_lf_pairs(const uint32_t *__restrict__ lf, const uint8_t *__restrict__ L,
+                                                       Tiling tl, uint32_t dshift, uint64_t *__restrict__ pair) {
+    const uint32_t b = tile_block(tl.first, tl.B, blockIdx.x);
+    const uint32_t tb = blockIdx.x - tl.first[b];
+    const BlockInfo bi = tl.blocks[b];
+    const uint32_t dmask = (1u << dshift) - 1u;
+    const size_t beg = (size_t)bi.off + (size_t)tb * LF_TILE;
+    const size_t end = (size_t)bi.off + bi.n;
+    uint32_t v[LF_ITEMS];
+#pragma unroll
+    for (int k = 0; k < LF_ITEMS; k++) {
+        const size_t q = beg + (size_t)k * LF_BLOCK + threadIdx.x;
+        v[k] = q < end ? lf[q] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < LF_ITEMS; k++) {
+        const size_t q = beg + (size_t)k * LF_BLOCK + threadIdx.x;
+        if (q < end) {
+            const uint32_t row = v[k] - bi.roff;
+            const bool stop = (row & dmask) == 0 || row == bi.P;
+            const uint32_t x = stop ? v[k] : lf[bi.off + row - (row < bi.P ? 0u : 1u)];
+            pair[q] = ((uint64_t)stop << 40) | ((uint64_t)L[q] << 32) | x;
+        }
+    }
+}
+
 // Splitters of block b: local j < nreg is the row j*D, j == nreg the row P (only when P % D != 0); global id
 // spoff + j. From each, walk LF to the next splitter row of the block; link = (steps << 32) | next splitter, len =
 // steps. The bytes met on the way (the text backwards from the splitter's suffix) are staged step-major,
-// stage[t * sstride + id] for the first cap steps t, so that the lanes of a wave, which walk consecutive splitters
-// in step, store consecutive bytes; cont keeps the local row of step cap of a longer sublist. The marker row P ends
-// the list: link = (0, S). Workgroups take WALK_TILE splitters of one block.
+// stage[t * sstride + id] for the first cap steps t (cap even), so that the lanes of a wave, which walk consecutive
+// splitters in step, store consecutive bytes; cont keeps the local row of step cap of a longer sublist. The marker
+// row P ends the list: link = (0, S). Workgroups take WALK_TILE splitters of one block.
 //
-// The walk is a chain of dependent loads, so its time is the longest sublist times the load latency: the load of
-// the next row is issued before the current row's byte is looked up and stored.
-__global__ __launch_bounds__(WALK_TILE) void k_walk_count(const uint32_t *__restrict__ lf, Tiling tl, uint32_t dshift,
+// The walk is a chain of dependent loads, so its time is the longest sublist times the load latency. It moves two
+// steps per load through the pair table (one when the first step lands on a splitter), and the next load is issued
+// before the current bytes are looked up and stored.
+__global__ __launch_bounds__(WALK_TILE) void k_walk_count(const uint64_t *__restrict__ pair, Tiling tl, uint32_t dshift,
                                                           uint32_t S, const uint32_t *__restrict__ cbase,
                                                           uint64_t *__restrict__ link, uint32_t *__restrict__ len,
                                                           uint8_t *__restrict__ stage, size_t sstride, uint32_t cap,
@@ -765,21 +797,30 @@ __global__ __launch_bounds__(WALK_TILE) void k_walk_count(const uint32_t *__rest
     const uint32_t r0 = j < bi.nreg ? (j << dshift) : bi.P;
     if (r0 == bi.P) { link[id] = S; len[id] = 0; return; }
     uint32_t steps = 0, r = r0;
-    uint32_t v = lf[bi.off + r0 - (r0 < bi.P ? 0u : 1u)];   // LF of the row of step 0
+    uint64_t e = pair[bi.off + r0 - (r0 < bi.P ? 0u : 1u)];
     uint8_t *st = stage + id;
     for (;;) {
-        const uint32_t rn = v - bi.roff;                       // the row of step steps + 1
-        const bool more = (rn & dmask) != 0 && rn != bi.P && steps + 1 <= bi.n;
-        uint32_t vn = 0;
-        if (more) {
-            if (steps + 1 == cap) cont[id] = rn;
-            vn = lf[bi.off + rn - (rn < bi.P ? 0u : 1u)];
+        const uint32_t x = (uint32_t)e;
+        const uint8_t b1 = (uint8_t)(e >> 32);
+        if ((e >> 40) & 1u) {                                  // the first step lands on a splitter
+            if (steps < cap) st[(size_t)steps * sstride] = b1;
+            steps++;
+            r = x - bi.roff;
+            break;
         }
-        if (steps < cap) st[(size_t)steps * sstride] = row_byte(s_c, v);
-        steps++;
+        const uint32_t rn = x - bi.roff;                       // the row after two steps
+        const bool more = (rn & dmask) != 0 && rn != bi.P && steps + 2 <= bi.n;
+        uint64_t en = 0;
+        if (more) {
+            if (steps + 2 == cap) cont[id] = rn;
+            en = pair[bi.off + rn - (rn < bi.P ? 0u : 1u)];
+        }
+        if (steps < cap) st[(size_t)steps * sstride] = b1;
+        if (steps + 1 < cap) st[(size_t)(steps + 1) * sstride] = row_byte(s_c, x);
+        steps += 2;
         r = rn;
         if (!more) break;
-        v = vn;
+        e = en;
     }
     const uint32_t nxt = r == bi.P ? sidP : bi.spoff + (r >> dshift);
     link[id] = ((uint64_t)steps << 32) | nxt;
@@ -1016,11 +1057,15 @@ __global__ void k_epoch_bump(uint32_t *depoch) {
     *depoch = (v & (EPOCH_HALF - 1)) | EPOCH_HALF;
 }
 void launch_epoch_bump(uint32_t *depoch, hipStream_t s) { k_epoch_bump<<<1, 1, 0, s>>>(depoch); }
-void launch_walk_count(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
+void launch_lf_pairs(const uint32_t *lf, const uint8_t *L, Tiling tl, uint32_t ntiles, uint32_t dshift,
+                     uint64_t *pair, hipStream_t s) {
+    if (ntiles) k_lf_pairs<<<ntiles, LF_BLOCK, 0, s>>>(lf, L, tl, dshift, pair);
+}
+void launch_walk_count(const uint64_t *pair, Tiling tl, uint32_t ntiles, uint32_t dshift, uint32_t S,
                        const uint32_t *cbase, uint64_t *link, uint32_t *len, uint8_t *stage, size_t sstride,
                        uint32_t cap, uint32_t *cont, hipStream_t s) {
     if (ntiles)
-        k_walk_count<<<ntiles, WALK_TILE, 0, s>>>(lf, tl, dshift, S, cbase, link, len, stage, sstride, cap, cont);
+        k_walk_count<<<ntiles, WALK_TILE, 0, s>>>(pair, tl, dshift, S, cbase, link, len, stage, sstride, cap, cont);
 }
 uint32_t rank_jump_iters(uint32_t S) {
     // reach after k steps: RANK_HOPS^k list nodes; a list has at most S nodes
