@@ -54,7 +54,7 @@ struct Workspace {
     std::mutex mu;
     uint32_t epoch = 1;
     Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, st_max, samples, stage, blk, counts, dbase,
-        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es, ecnt, eoff, pair;
+        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es, ecnt, eoff, pair, tmax;
     uint32_t *counters = nullptr;           // 16 tile counters (zeroed once); [15]: the device epoch of graphs
     uint32_t *gcnt = nullptr;               // 4 group-list counters
     uint32_t *h_gcnt = nullptr;
@@ -526,6 +526,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     grow(w.es, (NN / SEG_TILE + 2) * SEG_TILE * 4);
     grow(w.ecnt, (NN / SEG_TILE + 2) * 4);
     grow(w.eoff, (NN / SEG_TILE + 2) * 4);
+    grow(w.tmax, (NN / SEG_TILE + 2) * 4);
     grow(w.sec, NN * 4);
     grow(w.sufl, NN * 4);
     grow(w.lo, GMAX * 8);
@@ -596,6 +597,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         sa.nh = P<uint32_t>(w.eh);
         sa.ns = P<uint32_t>(w.es);
         sa.ncnt = P<uint32_t>(w.ecnt);
+        sa.tmax = P<uint32_t>(w.tmax);
         sa.rank_max = rank_max;
         if (Gs) TIMED("seg_gather", Es, launch_seg_round(0, sa, s));
         Shifts sh{};
@@ -628,7 +630,8 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         // sort and split; the next round's lists are built from zero
         zero_counts();
         if (Gs) {
-            TIMED("seg_sort", Es, launch_seg_round(1, sa, s));
+            // the split: direct ranking for tiles whose splitting groups are small, the LDS radix sort for the rest
+            TIMED("seg_sort", Es, launch_seg_round(2, sa, s); launch_seg_round(1, sa, s));
             // the tiles' emissions into the next small list; its length goes to gcnt[0], where the large groups'
             // subgroups are appended after it
             const uint32_t nt = (Es + SEG_TILE - 1) / SEG_TILE;
@@ -1363,7 +1366,7 @@ void kbwt_release(void) {
                    &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.st_max, &w.samples,
                    &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
                    &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
-                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,  &w.sufl,     &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair};
+                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,  &w.sufl,     &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax};
     if (w.inv_graph.exec) {
         (void)hipGraphExecDestroy(w.inv_graph.exec);
         w.inv_graph.exec = nullptr;

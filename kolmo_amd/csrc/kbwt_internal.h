@@ -103,6 +103,7 @@ struct SegRoundArgs {
     const BlockInfo *blocks;      // every block (blocks[0] when blk == nullptr)
     uint32_t n, round;            // doubling round >= 1: offset round_offset(block, round)
     uint32_t *nh, *ns, *ncnt;     // the split's emissions: per tile, SEG_TILE entries at tile * SEG_TILE, and counts
+    uint32_t *tmax;               // per tile: rows of its largest group (written by the gather)
     uint32_t rank_max;            // tiles whose groups have at most rank_max rows rank them directly
 };
 

@@ -178,9 +178,14 @@ __global__ __launch_bounds__(256) void k_hist_T(const uint8_t *__restrict__ T, T
     // most HT_FLUSH tiles / HT_COPIES rows between flushes, so no half overflows.
     constexpr int HT_COPIES = 8, HT_FLUSH = 64;
     static_assert((size_t)HT_FLUSH * HIST_TILE / HT_COPIES < 65536, "16-bit bins");
+    // Thread t keys the RUN consecutive rows from t * RUN, sliding its key one code per row. The codes sit in LDS
+    // with 4 spare bytes after every 16, so the runs of a wave's lanes start in different banks.
+    constexpr int RUN = HIST_TILE / 256;
+    static_assert(RUN == 16, "one 16-byte chunk per thread");
     __shared__ uint32_t h[HT_COPIES][MAXP][128];
     __shared__ uint8_t s_code[256];
-    __shared__ uint8_t s_c[HIST_TILE + 64];
+    __shared__ uint8_t s_c[(HIST_TILE + 64) / 16 * 20];
+    auto at = [](uint32_t j) { return j + ((j >> 4) << 2); };
     const int tid = threadIdx.x, cp = tid % HT_COPIES;
     for (int i = tid; i < HT_COPIES * MAXP * 128; i += 256) (&h[0][0][0])[i] = 0;
     const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
@@ -214,14 +219,21 @@ __global__ __launch_bounds__(256) void k_hist_T(const uint8_t *__restrict__ T, T
         }
         since++;
         stage_bytes<256>(T, r.beg, HIST_TILE + h0, r.bend,
-                         [&](uint32_t j, uint8_t t) { s_c[j] = r.beg + j < r.bend ? s_code[t] : 0; });
+                         [&](uint32_t j, uint8_t t) { s_c[at(j)] = r.beg + j < r.bend ? s_code[t] : 0; });
         __syncthreads();
-        for (uint32_t j = tid; j < HIST_TILE; j += 256) {
-            if (r.beg + j >= r.end) break;
-            const uint64_t key = build_key(s_c + j, 0, h0, w);
-            for (uint32_t p = 0; p < P; p++) {
-                const uint32_t d = (uint32_t)(key >> (8 * p + 8)) & 255u;
-                atomicAdd(&h[cp][p][d >> 1], 1u << (16 * (d & 1)));
+        const uint32_t j0 = (uint32_t)tid * RUN;
+        const uint32_t nrow = r.end - r.beg;
+        if (j0 < nrow) {
+            const uint64_t kmask = (1ull << (h0 * w)) - 1ull;   // h0 * w <= 56
+            uint64_t k = 0;
+            for (uint32_t c = 0; c + 1 < h0; c++) k = (k << w) | s_c[at(j0 + c)];
+            const uint32_t nr = nrow - j0 < (uint32_t)RUN ? nrow - j0 : (uint32_t)RUN;
+            for (uint32_t i = 0; i < nr; i++) {
+                k = ((k << w) | s_c[at(j0 + i + h0 - 1)]) & kmask;   // the codes of rows j0 + i .. j0 + i + h0 - 1
+                for (uint32_t p = 0; p < P; p++) {
+                    const uint32_t d = (uint32_t)(k >> (8 * p)) & 255u;
+                    atomicAdd(&h[cp][p][d >> 1], 1u << (16 * (d & 1)));
+                }
             }
         }
     }
@@ -265,7 +277,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     constexpr int W = BLOCK / WAVE;
     constexpr int TILE = BLOCK * ITEMS;
     static_assert(BLOCK >= 256, "one thread per digit");
-    static_assert(TILE * 8 >= 2 * TILE + 128, "LDS staging of raw bytes and codes");
+    static_assert(TILE * 8 >= (TILE + 64) / 16 * 40, "LDS staging of raw bytes and codes");
     __shared__ uint32_t s_wcnt[W][256];
     __shared__ uint32_t s_tstart[256];
     __shared__ uint32_t s_gbase[256];
@@ -275,6 +287,11 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     __shared__ uint8_t s_code[FROM_T ? 256 : 1];
 
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+    // element of item k of this lane: k-major, lane-minor in a wave's run (stable), or ITEMS consecutive elements
+    // per lane in the first pass
+    auto item = [&](int k) -> uint32_t {
+        return FROM_T ? (uint32_t)(wave * WAVE + lane) * ITEMS + k : (uint32_t)(wave * WAVE * ITEMS + k * WAVE + lane);
+    };
     if (tid == 0) s_tile = next_tile(counter, ntiles);
     for (int i = tid; i < W * 256; i += BLOCK) (&s_wcnt[0][0])[i] = 0;
     __syncthreads();
@@ -287,24 +304,32 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     uint64_t key[ITEMS];
     uint32_t val[ITEMS];
     if (FROM_T) {
-        // raw[j] = T[base + j - 1] (the BWT byte of element j), cod[j] = code of T[base + j], zero past the block
+        // raw[j] = T[base + j - 1] (the BWT byte of element j), cod[j] = code of T[base + j], zero past the block.
+        // The first pass need not be stable (its input order is text order, which nothing depends on), so a lane keys
+        // ITEMS consecutive elements, sliding its key one code per element; 4 spare bytes after every 16 put the
+        // lanes' runs in different LDS banks.
         const uint32_t w = tl.blocks[tr.b].w, h0 = tl.blocks[tr.b].h0;
         const size_t boff = tl.blocks[tr.b].off;
         if (tid < 256) s_code[tid] = codes[(size_t)tr.b * 256 + tid];
+        auto at = [](uint32_t j) { return j + ((j >> 4) << 2); };
         uint8_t *s_raw = reinterpret_cast<uint8_t *>(s_buf);
-        uint8_t *s_cod = s_raw + TILE + 64;
+        uint8_t *s_cod = s_raw + (TILE + 64) / 16 * 20;
         __syncthreads();
         stage_bytes<BLOCK>(T, base, (uint32_t)TILE + h0, tr.bend, [&](uint32_t j, uint8_t t) {
-            s_cod[j] = base + j < tr.bend ? s_code[t] : 0;
-            if (j + 1 <= (uint32_t)TILE) s_raw[j + 1] = t;
+            s_cod[at(j)] = base + j < tr.bend ? s_code[t] : 0;
+            if (j + 1 <= (uint32_t)TILE) s_raw[at(j + 1)] = t;
         });
         if (tid == 0) s_raw[0] = base > boff ? T[base - 1] : 0;
         __syncthreads();
+        const uint32_t j0 = (wave * WAVE + lane) * ITEMS;
+        const uint64_t kmask = (1ull << (h0 * w)) - 1ull;   // h0 * w <= 56
+        uint64_t kp = 0;
+        for (uint32_t c = 0; c + 1 < h0; c++) kp = (kp << w) | s_cod[at(j0 + c)];
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
-            uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
-            key[k] = build_key(s_cod + local, s_raw[local], h0, w);
-            val[k] = (uint32_t)(base + local);
+            kp = ((kp << w) | s_cod[at(j0 + k + h0 - 1)]) & kmask;
+            key[k] = (kp << 8) | s_raw[at(j0 + k)];
+            val[k] = (uint32_t)(base + j0 + k);
         }
         __syncthreads();   // s_buf is reused below
     } else {
@@ -316,12 +341,11 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
         }
     }
 
-    // stable ranking inside each wave, items in element order (k-major, lane-minor)
+    // ranking inside each wave in (k, lane) order: stable when that is element order
     uint32_t rk[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
-        bool valid = local < cnt;
+        bool valid = item(k) < cnt;
         uint32_t d = (uint32_t)(key[k] >> shift) & 255u;
         uint64_t peers = match_digit8(d, __ballot(valid));
         rk[k] = 0;
@@ -362,10 +386,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
         lidx[k] = s_tstart[d] + s_wcnt[wave][d] + rk[k];
     }
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
-        if (local < cnt) s_buf[lidx[k]] = key[k];
-    }
+    for (int k = 0; k < ITEMS; k++)
+        if (item(k) < cnt) s_buf[lidx[k]] = key[k];
     __syncthreads();
     uint32_t dg[ITEMS];
 #pragma unroll
@@ -382,10 +404,8 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     __syncthreads();
     uint32_t *s_v = reinterpret_cast<uint32_t *>(s_buf);
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
-        if (local < cnt) s_v[lidx[k]] = val[k];
-    }
+    for (int k = 0; k < ITEMS; k++)
+        if (item(k) < cnt) s_v[lidx[k]] = val[k];
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {

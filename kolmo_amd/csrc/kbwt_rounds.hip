@@ -157,10 +157,14 @@ __device__ __forceinline__ uint32_t sec_of(const uint32_t *ISA, uint32_t i, uint
     return j < end ? ISA[j] + h : end - 1u - i;
 }
 
-// One tile of small groups. MODE 0 gathers every row's key; MODE 1 sorts each group by it in LDS, rewrites SA, ISA
-// and the BWT bytes of its rows, and writes the subgroups of two or more into the tile's own area of the emission
-// buffers (nh, ns at tile * SEG_TILE; ncnt[tile] counts them), which k_compact_emits packs into the next list. One
-// shared counter for all tiles would serialise every workgroup on a single atomic.
+// One tile of small groups. MODE 0 gathers every row's key and records in tmax the rows of the tile's largest group
+// that splits (whose keys are not all equal; 0 when none does). The split sorts each group by its key in LDS,
+// rewrites SA, ISA and the BWT bytes of its rows, and writes the subgroups of two or more into the tile's own area of
+// the emission buffers (nh, ns at tile * SEG_TILE; ncnt[tile] counts them), which k_compact_emits packs into the next
+// list (one shared counter for all tiles would serialise every workgroup on a single atomic). MODE 2 splits the
+// tiles whose splitting groups have at most rank_max rows (empty tiles and tiles where nothing splits included) by
+// direct ranking; its small LDS footprint lets eight workgroups share a CU. MODE 1 splits the other tiles by an LDS
+// radix sort.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ gh, const uint32_t *__restrict__ go,
                                                    const uint32_t *__restrict__ tstart, uint32_t G, uint32_t E, uint32_t *__restrict__ SA,
@@ -170,20 +174,24 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
                                                    const uint16_t *__restrict__ blk,
                                                    const BlockInfo *__restrict__ blocks, uint32_t n, uint32_t round,
                                                    uint32_t *__restrict__ nh, uint32_t *__restrict__ ns,
-                                                   uint32_t *__restrict__ ncnt, uint32_t rank_max) {
+                                                   uint32_t *__restrict__ ncnt, uint32_t *__restrict__ tmax,
+                                                   uint32_t rank_max) {
     // a tile's rows: the groups starting in its SEG_TILE rows, each of at most SEG_GROUP rows. CAP keeps the LDS of
-    // a workgroup within a quarter of a CU's, so four workgroups share a CU (the rounds are latency-bound)
+    // a radix workgroup within a quarter of a CU's, so four share a CU (the rounds are latency-bound)
     constexpr int CAP = SEG_TILE + SEG_GROUP;
     constexpr int MAXG = SEG_TILE / 2;         // groups starting in the tile (two rows or more each)
     constexpr int ITEMS = CAP / 256;
     constexpr int W = 4;
+    constexpr bool RADIX = MODE == 1;
     __shared__ uint32_t s_gh[MAXG], s_go[MAXG];
     __shared__ uint32_t s_hoff[MODE ? 1 : MAXG], s_gend[MODE ? 1 : MAXG];   // gather: each group's offset and block end
-    __shared__ uint32_t s_g0, s_g1, s_cnt, s_maxsz;
-    __shared__ uint64_t s_buf[MODE ? 2 * CAP : 1];
+    __shared__ uint32_t s_first[RADIX ? 1 : MAXG];   // key of each group's first row; the ranking: whether it splits
+    __shared__ uint32_t s_g0, s_g1, s_cnt, s_maxsz, s_skip;
+    __shared__ uint64_t s_buf[RADIX ? 2 * CAP : 1];
+    __shared__ uint32_t s_sec[MODE == 2 ? CAP : 1];
     __shared__ uint32_t s_i[MODE ? CAP : 1];
-    __shared__ uint32_t s_wcnt[W][256];
-    __shared__ uint32_t s_start[256];
+    __shared__ uint32_t s_wcnt[RADIX ? W : 1][256];
+    __shared__ uint32_t s_start[RADIX ? 256 : 1];
     __shared__ uint32_t s_scratch[16];
     __shared__ uint64_t s_or[W], s_and[W];
 
@@ -192,11 +200,19 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         s_g0 = tstart[blockIdx.x];
         s_g1 = tstart[blockIdx.x + 1];
         s_maxsz = 0;
+        if (MODE) {
+            const uint32_t mx = tmax[blockIdx.x];
+            s_skip = RADIX ? mx <= rank_max : mx > rank_max;   // the other split kernel takes this tile
+        }
     }
     __syncthreads();
+    if (MODE && s_skip) return;
     const uint32_t g0 = s_g0, g1 = s_g1, ng = g1 - g0;
     if (ng == 0) {
-        if (MODE == 1 && tid == 0) ncnt[blockIdx.x] = 0;
+        if (tid == 0) {
+            if (MODE == 0) tmax[blockIdx.x] = 0;
+            else ncnt[blockIdx.x] = 0;
+        }
         return;
     }
     const uint32_t span0 = go[g0], span1 = g1 < G ? go[g1] : E;
@@ -212,8 +228,6 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         }
     }
     __syncthreads();
-    if (MODE == 1)
-        for (uint32_t j = tid; j < ng; j += 256) atomicMax(&s_maxsz, (j + 1 < ng ? s_go[j + 1] : cnt) - s_go[j]);
 
     // gather: row e of the tile belongs to group lg (binary search over the tile's group offsets). MODE 0 reads each
     // row's suffix from SA and its key at i + h, and stores both in list order (row span0 + e of the round: sufl,
@@ -237,8 +251,13 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
             gi[k] = e < cnt ? SA[s_gh[ga[k]] + (e - s_go[ga[k]])] : 0u;
         }
 #pragma unroll
-        for (int k = 0; k < ITEMS; k++)
-            if (tid + k * 256 < cnt) gv[k] = sec_of(ISA, gi[k], s_hoff[ga[k]], s_gend[ga[k]]);
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t e = tid + k * 256;
+            if (e < cnt) {
+                gv[k] = sec_of(ISA, gi[k], s_hoff[ga[k]], s_gend[ga[k]]);
+                if (e == s_go[ga[k]]) s_first[ga[k]] = gv[k];
+            }
+        }
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             const uint32_t e = tid + k * 256;
@@ -247,6 +266,14 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
                 sec[span0 + e] = gv[k];
             }
         }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t e = tid + k * 256, lg = ga[k];
+            if (e < cnt && gv[k] != s_first[lg]) atomicMax(&s_maxsz, (lg + 1 < ng ? s_go[lg + 1] : cnt) - s_go[lg]);
+        }
+        __syncthreads();
+        if (tid == 0) tmax[blockIdx.x] = s_maxsz;
         return;
     }
 #pragma unroll
@@ -255,46 +282,53 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         gi[k] = e < cnt ? sufl[span0 + e] : 0u;
         gv[k] = e < cnt ? sec[span0 + e] : 0u;
     }
-    uint64_t ko = 0, ka = ~0ull;
-#pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        const uint32_t e = tid + k * 256;
-        if (e < cnt) {
-            const uint64_t key = (((uint64_t)ga[k] << 32) | gv[k]) << 12 | e;   // group, sec, row (sort as a unit)
-            s_buf[e] = key;
-            s_i[e] = gi[k];
-            ko |= key;
-            ka &= key;
-        }
-    }
-    __syncthreads();
 
-    // Tiles of small groups only: each row counts, within its group, the rows of smaller sec (less) and of equal sec
-    // before it (eqb). Its new SA position is head_of_group + less + eqb, its subgroup's head head_of_group + less,
-    // and the last row of a subgroup of two or more emits it. Same order as the radix path below (rows of equal sec
-    // keep their slot order). Both paths write nothing for a group that does not split, and the BWT byte of a row
-    // only when it ends up alone (its SA position is final then).
-    if (s_maxsz <= rank_max) {
-        constexpr int PERR = CAP / 256;
-        uint32_t eh[PERR], es[PERR];
+    // Direct ranking: each row counts, within its group, the rows of smaller sec (less) and of equal sec before it
+    // (eqb). Its new SA position is head_of_group + less + eqb, its subgroup's head head_of_group + less, and the
+    // last row of a subgroup of two or more emits it. Same order as the radix split (rows of equal sec keep their
+    // slot order). Both write nothing for a group that does not split, and the BWT byte of a row only when it ends
+    // up alone (its SA position is final then).
+    if (MODE == 2) {
+        // a group splits when some row's key differs from its first row's; the rows of a group that does not
+        // split know their ranks without comparing
+        for (uint32_t j = tid; j < ng; j += 256) s_first[j] = 0;
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t e = tid + k * 256;
+            if (e < cnt) {
+                s_sec[e] = gv[k];
+                s_i[e] = gi[k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t e = tid + k * 256;
+            if (e < cnt && gv[k] != s_sec[s_go[ga[k]]]) s_first[ga[k]] = 1;
+        }
+        __syncthreads();
+        uint32_t eh[ITEMS], es[ITEMS];
         uint32_t nemit = 0;
 #pragma unroll
-        for (int k = 0; k < PERR; k++) {
+        for (int k = 0; k < ITEMS; k++) {
             const uint32_t e = tid + k * 256;
             es[k] = 0;
             eh[k] = 0;
             if (e < cnt) {
-                const uint64_t key = s_buf[e] >> 12;
-                const uint32_t lg = (uint32_t)(key >> 32), se = (uint32_t)key;
+                const uint32_t lg = ga[k], se = gv[k];
                 const uint32_t st = s_go[lg], en = lg + 1 < ng ? s_go[lg + 1] : cnt;
-                uint32_t less = 0, eq = 0, eqb = 0;
-                for (uint32_t j = st; j < en; j++) {
-                    const uint32_t sj = (uint32_t)(s_buf[j] >> 12);
-                    less += sj < se ? 1u : 0u;
-                    eq += sj == se ? 1u : 0u;
-                    eqb += (sj == se && j < e) ? 1u : 0u;
+                uint32_t less = 0, eq = en - st, eqb = e - st;
+                if (s_first[lg]) {
+                    eq = 0;
+                    eqb = 0;
+                    for (uint32_t j = st; j < en; j++) {
+                        const uint32_t sj = s_sec[j];
+                        less += sj < se ? 1u : 0u;
+                        eq += sj == se ? 1u : 0u;
+                        eqb += (sj == se && j < e) ? 1u : 0u;
+                    }
                 }
-                const uint32_t i = s_i[e];
+                const uint32_t i = gi[k];
                 const uint32_t head = s_gh[lg] + less, pnew = head + eqb;
                 if (eq != en - st) {             // the group splits (else SA, ISA stay as they are)
                     // only what changes: a row that keeps its SA position or its head (the first subgroup keeps
@@ -311,10 +345,24 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         if (tid == 0) ncnt[blockIdx.x] = total_emit;
         size_t at = (size_t)blockIdx.x * SEG_TILE + my_emit;
 #pragma unroll
-        for (int k = 0; k < PERR; k++)
+        for (int k = 0; k < ITEMS; k++)
             if (es[k]) { nh[at] = eh[k]; ns[at] = es[k]; at++; }
         return;
     }
+
+    uint64_t ko = 0, ka = ~0ull;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const uint32_t e = tid + k * 256;
+        if (e < cnt) {
+            const uint64_t key = (((uint64_t)ga[k] << 32) | gv[k]) << 12 | e;   // group, sec, row (sort as a unit)
+            s_buf[e] = key;
+            s_i[e] = gi[k];
+            ko |= key;
+            ka &= key;
+        }
+    }
+    __syncthreads();
 
     // which key bits vary over the tile (bits 12 and up: the row index below is carried, not sorted)
     for (int o = 1; o < WAVE; o <<= 1) {
@@ -483,12 +531,13 @@ void launch_large_elems(const uint32_t *lh, const uint32_t *lo, uint32_t G, uint
 void launch_seg_round(int mode, const SegRoundArgs &a, hipStream_t s) {
     const uint32_t ntiles = (a.E + SEG_TILE - 1) / SEG_TILE;
     if (!ntiles) return;
-    if (mode == 0)
-        k_seg_round<0><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.sufl, a.L, a.T, a.blk, a.blocks, a.n,
-                                              a.round, a.nh, a.ns, a.ncnt, a.rank_max);
-    else
-        k_seg_round<1><<<ntiles, 256, 0, s>>>(a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.sufl, a.L, a.T, a.blk, a.blocks, a.n,
-                                              a.round, a.nh, a.ns, a.ncnt, a.rank_max);
+#define KBWT_SEG_ARGS a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.sufl, a.L, a.T, a.blk, a.blocks, a.n, a.round, \
+                      a.nh, a.ns, a.ncnt, a.tmax, a.rank_max
+    // every tile goes to exactly one of the two split kernels (modes 1 and 2)
+    if (mode == 0) k_seg_round<0><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
+    else if (mode == 1) k_seg_round<1><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
+    else k_seg_round<2><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
+#undef KBWT_SEG_ARGS
 }
 
 }  // namespace kbwt
