@@ -44,6 +44,7 @@ KERNEL_BYTES = {
     "round_keys": 32.0,    # read pos, grp, SA, ISA[i+h] (4 each), write key 8 + value 4, plus the active row's SA read
     "bwt": 2.0,            # read the row byte, write the BWT byte
     "lf_pass": 5.0,        # read L 1, write LF 4
+    "lf_pairs": 17.0,      # read LF 4 and L 1, gather LF(LF) 4, write the pair 8
     "walk_count": 5.0,     # one LF read and one staged byte per step
     "walk_emit": 2.0,      # one staged byte read and written per step
     "rank_jump": 24.0,     # read two links, write one
