@@ -5,6 +5,8 @@ Test infrastructure only: the product never loads these.
   ref()          kolmo's block-sorting path as built from the kolmo tree (sais.c and its codec glue, unchanged)
   codec(which)   kolmo's codec glue (bwt_stream.c, lzp.c, qlfc.c, par.c, klzma.c) with sais.c replaced:
                  which = "oracle" (over the oracle) or "hip" (over libkbwt_hip.so, the drop-in relink)
+  full(which)    the whole kolmo library (kolmo_compress, kolmo_decompress): which = "ref" (unchanged), "oracle"
+                 or "hip" (sais.c replaced as above)
 """
 import ctypes
 import hashlib
@@ -86,6 +88,57 @@ def codec(which):
         _codec_sigs(L)
         _cache[key] = L
     return _cache[key]
+
+
+class KolmoParams(ctypes.Structure):
+    """kolmo_params (src/kolmo.h:21-41)."""
+    _fields_ = [("mode", ctypes.c_int), ("level", ctypes.c_int), ("threads", ctypes.c_int), ("block", ctypes.c_uint32),
+                ("transforms", ctypes.c_int), ("mem_limit", ctypes.c_uint64)]
+
+
+def have_full(which):
+    return os.path.exists(os.path.join(REFDIR, f"libkolmo_full_{which}.so"))
+
+
+def full(which):
+    key = "full_" + which
+    if key not in _cache:
+        L = ctypes.CDLL(os.path.join(REFDIR, f"libkolmo_full_{which}.so"))
+        L.kolmo_params_for_level.restype = ctypes.c_int
+        L.kolmo_params_for_level.argtypes = [ctypes.c_int, ctypes.POINTER(KolmoParams)]
+        L.kolmo_bound.restype = ctypes.c_size_t
+        L.kolmo_bound.argtypes = [ctypes.c_size_t, ctypes.POINTER(KolmoParams)]
+        L.kolmo_compress.restype = ctypes.c_int64
+        L.kolmo_compress.argtypes = [_u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t, ctypes.POINTER(KolmoParams)]
+        L.kolmo_decompress.restype = ctypes.c_int64
+        L.kolmo_decompress.argtypes = [_u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t, ctypes.c_int]
+        if which == "oracle":
+            L.oracle_shim_calls.restype = ctypes.c_int64
+            L.oracle_shim_calls.argtypes = [ctypes.c_int32]
+        _cache[key] = L
+    return _cache[key]
+
+
+def kolmo_compress(L, src, level, threads=1, block=0):
+    """A whole kolmo frame of src at a level (kolmo_params_for_level), with the given threads and block size."""
+    src = as_u8(src)
+    p = KolmoParams()
+    assert L.kolmo_params_for_level(level, ctypes.byref(p)) == 0
+    p.threads = threads
+    p.block = block
+    cap = L.kolmo_bound(src.size, ctypes.byref(p))
+    out = np.empty(cap, np.uint8)
+    k = L.kolmo_compress(_p(src), src.size, _p(out), cap, ctypes.byref(p))
+    assert k > 0, k
+    return out[:k]
+
+
+def kolmo_decompress(L, frame, n, threads=1):
+    frame = as_u8(frame)
+    out = np.empty(max(n, 1), np.uint8)
+    k = L.kolmo_decompress(_p(frame), frame.size, _p(out), out.size, threads)
+    assert k == n, k
+    return out[:n]
 
 
 def as_u8(x):
