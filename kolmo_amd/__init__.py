@@ -346,7 +346,7 @@ class Tuning(ctypes.Structure):
                 ("ms", ctypes.c_float)]
 
 
-DEFAULT_TUNING = (0, 6, 16)
+DEFAULT_TUNING = (8, 6, 16)
 
 
 def _tune_sigs(L):

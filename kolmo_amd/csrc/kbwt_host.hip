@@ -275,7 +275,7 @@ uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi) {
 // Launch parameters per size class (ceil(log2 N) of a call's packed length), set by kbwt_autotune or
 // kbwt_set_tuning. An environment variable (KBWT_SORT_CFG, KBWT_INV_DSHIFT, KBWT_SEG_RANK) overrides the table.
 struct Tuning {
-    uint32_t sort_cfg = 0, inv_dshift = 6, seg_rank = SEG_RANK_MAX;
+    uint32_t sort_cfg = 8, inv_dshift = 6, seg_rank = SEG_RANK_MAX;
 };
 constexpr int TUNE_CLASSES = 33;
 std::mutex g_tune_mu;
@@ -378,7 +378,8 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     if (B > 1) grow(w.blk, NN * 2);
 
     // metadata layout: blocks | hist tiles | sort tiles | codes; the byte counts come back behind it
-    const size_t o_ht = align16(B * sizeof(BlockInfo)), o_st = o_ht + (B + 1) * 4, o_codes = align16(o_st + (B + 1) * 4);
+    const size_t o_ht = align16(B * sizeof(BlockInfo)), o_st = o_ht + (B + 1) * 4, o_pt = o_st + (B + 1) * 4;
+    const size_t o_codes = align16(o_pt + (B + 1) * 4);
     const size_t bytes = o_codes + (size_t)B * 256;
     uint8_t *hm = (uint8_t *)host_staging(w, bytes + (size_t)B * 256 * 4);
     grow(w.meta, bytes);
@@ -390,9 +391,11 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     const uint8_t *d_codes = dm + o_codes;
     const uint32_t nht = fill_tiles((uint32_t *)(hm + o_ht), B, [&](uint32_t b) { return (bl[b].n + HIST_TILE - 1) / HIST_TILE; });
     const uint32_t nst = fill_tiles((uint32_t *)(hm + o_st), B, [&](uint32_t b) { return (bl[b].n + stile - 1) / stile; });
+    const uint32_t npt = fill_tiles((uint32_t *)(hm + o_pt), B, [&](uint32_t b) { return (bl[b].n + PART_TILE - 1) / PART_TILE; });
     memcpy(h_bl, bl.data(), B * sizeof(BlockInfo));
     HIPCHK(hipMemcpyAsync(dm, hm, o_codes, hipMemcpyHostToDevice, s));
     const Tiling thist{(const uint32_t *)(dm + o_ht), d_bl, B}, tsort{(const uint32_t *)(dm + o_st), d_bl, B};
+    const Tiling tpart{(const uint32_t *)(dm + o_pt), d_bl, B};
     const uint16_t *blk = B > 1 ? P<uint16_t>(w.blk) : nullptr;
 
     HIPCHK(hipEventRecord(w.ev0, s));
@@ -497,14 +500,13 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         g.epoch = next_epoch(w);
         TIMED("regroup", N, launch_regroup(g, s));
         LAUNCHCHK();
-        // ISA[suffix] = head: partition the pairs by suffix window (one keys-only radix pass), then write window by
-        // window instead of scattering 4-byte writes over all of ISA
-        const uint32_t wshift = isa_window_shift(N);
+        // ISA[suffix] = head: partition the pairs by suffix window inside each block (one keys-only radix pass),
+        // then write window by window instead of scattering 4-byte writes over all of ISA
         TIMED("isa_part", N,
-              launch_partition_pairs(K[P0 & 1], K[(P0 - 1) & 1], N, wshift, P<uint64_t>(w.status), w.counters + 4,
+              launch_partition_pairs(K[P0 & 1], K[(P0 - 1) & 1], N, tpart, npt, P<uint64_t>(w.status), w.counters + 4,
                                      next_epoch(w), s));
         LAUNCHCHK();
-        TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, wshift, ISA, s));
+        TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, ISA, s));
     }
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
@@ -690,7 +692,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
             g.st_max = P<uint64_t>(w.st_max);
             g.counter = w.counters + 1;
             g.epoch = next_epoch(w);
-            TIMED("regroup", ml, launch_regroup(g, s));
+                TIMED("regroup", ml, launch_regroup(g, s));
             TIMED("to_groups", ml,
                   launch_elems_to_groups(P<uint32_t>(w.pos[1]), P<uint32_t>(w.grp[1]), w.m_out, ml,
                                          P<uint32_t>(w.gh[cur ^ 1]), P<uint32_t>(w.gs[cur ^ 1]),

@@ -41,6 +41,9 @@ __device__ __forceinline__ uint32_t tile_block(const uint32_t *first, uint32_t B
     return lo;
 }
 
+// Smallest s with 256 << s >= n: the width of a block's ISA windows (at most 256 windows per block).
+__device__ __forceinline__ uint32_t window_shift(uint32_t n) { return n > 256 ? 24u - (uint32_t)__clz(n - 1) : 0u; }
+
 struct TileRange {
     uint32_t b;        // block
     uint32_t tb;       // tile within the block
@@ -264,8 +267,9 @@ __global__ __launch_bounds__(256) void k_hist_keys(const uint64_t *__restrict__ 
 // Stable pass over (key, value) pairs on the 8-bit digit at `shift`, segmented by block when tl.first is set.
 // FROM_T: keys are built from T with the block's code table (value = the suffix index). dbase[b*dstride + d]: start of
 // digit d in block b. status: one lookback word per (tile, digit); counter: tile-id counter.
-// Keys-only passes (VALS == false) move the 64-bit keys alone. dbase == nullptr gives every digit d the base
-// min(d << dstride, m): the bases of a partition of a permutation of [0, m) by its bits from dstride up.
+// Keys-only passes (VALS == false) move (suffix << 32 | head) pairs alone, partitioned by the suffix's window inside
+// its block b: digit (suffix - off_b) >> window_shift(n_b). The suffixes of a block are a permutation of its own
+// index range, so window d starts at off_b + min(d << window_shift(n_b), n_b) (dbase unused).
 template <int BLOCK, int ITEMS, int LB, bool FROM_T, bool VALS = true>
 __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                      uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
@@ -300,6 +304,15 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     const size_t base = tr.beg;
     const uint32_t cnt = (uint32_t)(tr.end - tr.beg);
     const uint32_t *db = dbase ? dbase + (size_t)tr.b * dstride : nullptr;
+    uint32_t woff = 0, wn = 0, wsh = 0;   // keys-only passes: the block's range and window width
+    if (!VALS) {
+        woff = tl.blocks[tr.b].off;
+        wn = tl.blocks[tr.b].n;
+        wsh = window_shift(wn);
+    }
+    auto digit = [&](uint64_t k) -> uint32_t {
+        return VALS ? (uint32_t)(k >> shift) & 255u : (((uint32_t)(k >> 32) - woff) >> wsh) & 255u;
+    };
 
     uint64_t key[ITEMS];
     uint32_t val[ITEMS];
@@ -346,7 +359,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         bool valid = item(k) < cnt;
-        uint32_t d = (uint32_t)(key[k] >> shift) & 255u;
+        uint32_t d = digit(key[k]);
         uint64_t peers = match_digit8(d, __ballot(valid));
         rk[k] = 0;
         if (valid) {
@@ -373,7 +386,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
             st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, ST_INC, excl + tot));
         }
         s_tstart[tid] = tstart;
-        const uint32_t dbv = db ? db[tid] : (uint32_t)min((uint64_t)tid << dstride, (uint64_t)m);
+        const uint32_t dbv = VALS ? db[tid] : woff + (uint32_t)min((uint64_t)tid << wsh, (uint64_t)wn);
         s_gbase[tid] = dbv + excl - tstart;
     }
     __syncthreads();
@@ -382,7 +395,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     uint32_t lidx[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        uint32_t d = (uint32_t)(key[k] >> shift) & 255u;
+        uint32_t d = digit(key[k]);
         lidx[k] = s_tstart[d] + s_wcnt[wave][d] + rk[k];
     }
 #pragma unroll
@@ -396,7 +409,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
         dg[k] = 0;
         if (j < cnt) {
             uint64_t kk = s_buf[j];
-            dg[k] = (uint32_t)(kk >> shift) & 255u;
+            dg[k] = digit(kk);
             kout[s_gbase[dg[k]] + j] = kk;
         }
     }
@@ -433,13 +446,15 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
                                                    uint32_t *__restrict__ grp_out, uint32_t *__restrict__ m_out,
                                                    uint64_t *__restrict__ st_sum, uint64_t *__restrict__ st_max,
                                                    uint32_t *__restrict__ counter, uint32_t epoch, uint32_t ntiles) {
-    // Rows move between HBM and the threads through LDS: coalesced loads and stores in row order, and each thread
-    // works on ITEMS consecutive rows, so one block-wide scan covers the tile. The LDS row of thread t starts at
+    // Rows come from HBM through LDS (coalesced loads in row order) and each thread works on ITEMS consecutive rows,
+    // so one block-wide scan covers the tile. The pairs leave through LDS in row order; the compacted rows (a
+    // thread's are consecutive in the output) are written from registers, which keeps the LDS footprint at four
+    // workgroups per CU. The LDS row of thread t starts at
     // t * (ITEMS + 1): the pad keeps a wave's lanes on distinct banks.
     constexpr int TILE = BLOCK * ITEMS, PITCH = ITEMS + 1;
+    static_assert(ITEMS == 8, "a thread's BWT bytes leave as one 64-bit word");
     __shared__ uint64_t s_k[BLOCK * PITCH];      // keys, later the (suffix, head) pairs
     __shared__ uint32_t s_v[BLOCK * PITCH];      // suffixes
-    __shared__ uint32_t s_cp[TILE], s_cg[TILE];  // the tile's compacted rows (position, head)
     __shared__ uint16_t s_b[BLOCK * PITCH];       // block ids (batches)
     __shared__ uint32_t s_scratch[32];
     __shared__ uint32_t s_imax[BLOCK];
@@ -530,7 +545,7 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
     // the head before this thread's first row: the tile's prefix and the threads before this one
     uint32_t head = s_ex_max;
     if (tid > 0 && s_imax[tid - 1] > head) head = s_imax[tid - 1];
-    uint32_t o = ex_s;                            // compacted rows before this thread's, within the tile
+    uint32_t o = s_ex_sum + ex_s;                 // compacted rows before this thread's
     uint64_t lbytes = 0;
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
@@ -544,9 +559,9 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
         const uint8_t c = T ? (i ? T[i - 1] : (uint8_t)0) : lbyte[k];
         if (pos) L[lp[k]] = c;
         else lbytes |= (uint64_t)c << (8 * k);
-        if (keeps & (1u << k)) {
-            s_cp[o] = lp[k];
-            s_cg[o] = head;
+        if (keeps & (1u << k)) {   // a thread's kept rows are consecutive in the output
+            pos_out[o] = lp[k];
+            grp_out[o] = head;
             o++;
         }
     }
@@ -556,41 +571,34 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
             for (int k = 0; k < ITEMS; k++)
                 if (r0 + k < cnt) L[t0 + r0 + k] = (uint8_t)(lbytes >> (8 * k));
     }
+    if (!pairs) return;
     __syncthreads();
-    // coalesced stores in row order: the pairs, then the compacted rows
-    if (pairs)
+    // coalesced stores of the pairs in row order
 #pragma unroll
-        for (int k = 0; k < ITEMS; k++) {
-            const uint32_t j = k * BLOCK + tid;
-            if (j < cnt) pairs[t0 + j] = s_k[at(j)];
-        }
-    const uint32_t base_out = s_ex_sum;
-    for (uint32_t j = tid; j < tot_s; j += BLOCK) {
-        pos_out[base_out + j] = s_cp[j];
-        grp_out[base_out + j] = s_cg[j];
+    for (int k = 0; k < ITEMS; k++) {
+        const uint32_t j = k * BLOCK + tid;
+        if (j < cnt) pairs[t0 + j] = s_k[at(j)];
     }
 }
 
 // ------------------------------------------------------------------ ISA from partitioned pairs
 
-// pairs: (suffix << 32 | head) partitioned by suffix >> wshift, so window w (suffixes [w << wshift, (w+1) << wshift))
-// holds exactly the pairs at the same indexes (the suffixes are a permutation of [0, N)). ISA[suffix] = head. The
-// writes of one window land in 4 * 2^wshift bytes of ISA; the windows are dealt to the workgroups so that window w is
-// written by the XCD w % 8 only (workgroups go to the XCDs round-robin), whose L2 merges them into whole lines.
-constexpr int SC_ITEMS = 16, SC_TILE = 256 * SC_ITEMS, NXCD = 8;
-__global__ __launch_bounds__(256) void k_isa_scatter(const uint64_t *__restrict__ pairs, uint32_t N, uint32_t wshift,
-                                                     uint32_t chunks, uint32_t NW, uint32_t *__restrict__ ISA) {
+// pairs: (suffix << 32 | head) partitioned by suffix window inside each block, so that every window occupies the
+// positions of its own suffix range (the suffixes of a block are a permutation of its index range): the pairs at
+// positions [p, q) carry suffixes of the windows that overlap [p, q). ISA[suffix] = head. A run of SC_SUPER tiles
+// therefore writes a short stretch of ISA, and each run is written by one XCD (workgroups are dealt to the XCDs
+// round-robin by id), whose L2 merges the writes into whole lines.
+constexpr int SC_ITEMS = 16, SC_TILE = 256 * SC_ITEMS, SC_SUPER = 32, NXCD = 8;
+__global__ __launch_bounds__(256) void k_isa_scatter(const uint64_t *__restrict__ pairs, uint32_t N,
+                                                     uint32_t *__restrict__ ISA) {
     const uint32_t x = blockIdx.x % NXCD, k = blockIdx.x / NXCD;
-    const uint32_t win = (k / chunks) * NXCD + x, chunk = k % chunks;
-    if (win >= NW) return;
-    const size_t w0 = (size_t)win << wshift;
-    const size_t wend = min(w0 + ((size_t)1 << wshift), (size_t)N);
-    const size_t b0 = w0 + (size_t)chunk * SC_TILE + threadIdx.x;
+    const size_t chunk = ((size_t)(k / SC_SUPER) * NXCD + x) * SC_SUPER + k % SC_SUPER;
+    const size_t b0 = chunk * SC_TILE + threadIdx.x;
     uint64_t v[SC_ITEMS];
 #pragma unroll
     for (int u = 0; u < SC_ITEMS; u++) {
         const size_t a = b0 + (size_t)u * 256;
-        v[u] = a < wend ? pairs[a] : ~0ull;
+        v[u] = a < N ? pairs[a] : ~0ull;
     }
 #pragma unroll
     for (int u = 0; u < SC_ITEMS; u++)
@@ -990,7 +998,8 @@ void launch_hist_keys(const uint64_t *K, uint32_t m, const Shifts &sh, uint32_t 
 
 // The radix pass configurations the tuner chooses from: (workgroup size, items per thread, lookback window).
 const SortCfg SORT_CFGS[SORT_NCFG] = {{512, 16, 4}, {256, 16, 4}, {256, 16, 1}, {512, 8, 4},
-                                      {256, 12, 4}, {256, 8, 4}, {512, 16, 1}};
+                                      {256, 12, 4}, {256, 8, 4}, {512, 16, 1}, {1024, 12, 4},
+                                      {1024, 16, 4}};
 uint32_t sort_tile(int cfg) { return (uint32_t)(SORT_CFGS[cfg].block * SORT_CFGS[cfg].items); }
 
 template <int BLOCK, int ITEMS, int LB>
@@ -1013,37 +1022,30 @@ void launch_sort_pass(const SortArgs &a, uint32_t ntiles, hipStream_t s) {
     case 4: sort_pass_cfg<256, 12, 4>(a, ntiles, s); break;
     case 5: sort_pass_cfg<256, 8, 4>(a, ntiles, s); break;
     case 6: sort_pass_cfg<512, 16, 1>(a, ntiles, s); break;
+    case 7: sort_pass_cfg<1024, 12, 4>(a, ntiles, s); break;
+    case 8: sort_pass_cfg<1024, 16, 4>(a, ntiles, s); break;
     default: sort_pass_cfg<512, 16, 4>(a, ntiles, s); break;
     }
 }
-void launch_partition_pairs(const uint64_t *in, uint64_t *out, uint32_t N, uint32_t wshift, uint64_t *status,
-                            uint32_t *counter, uint32_t epoch, hipStream_t s) {
-    constexpr int BLOCK = 512, ITEMS = 16;
-    const uint32_t ntiles = (uint32_t)(((size_t)N + BLOCK * ITEMS - 1) / (BLOCK * ITEMS));
+void launch_partition_pairs(const uint64_t *in, uint64_t *out, uint32_t N, const Tiling &tl, uint32_t ntiles,
+                            uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s) {
     if (!ntiles) return;
-    k_sort_pass<BLOCK, ITEMS, 4, false, false><<<ntiles, BLOCK, 0, s>>>(
-        in, nullptr, out, nullptr, N, 32 + wshift, nullptr, wshift, Tiling{nullptr, nullptr, 1}, status, counter,
-        epoch, ntiles, nullptr, nullptr);
+    k_sort_pass<PART_BLOCK, PART_ITEMS, 4, false, false><<<ntiles, PART_BLOCK, 0, s>>>(
+        in, nullptr, out, nullptr, N, 0, nullptr, 0, tl, status, counter, epoch, ntiles, nullptr, nullptr);
 }
-uint32_t isa_window_shift(uint32_t N) {
-    uint32_t b = 0;
-    while (((uint64_t)256 << b) < N) b++;   // at most 256 windows: one 8-bit digit
-    return b;
-}
-void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t wshift, uint32_t *ISA, hipStream_t s) {
+void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t *ISA, hipStream_t s) {
     if (!N) return;
-    const uint32_t NW = (uint32_t)(((uint64_t)N + ((uint64_t)1 << wshift) - 1) >> wshift);
-    const uint32_t chunks = (uint32_t)((((uint64_t)1 << wshift) + SC_TILE - 1) / SC_TILE);
-    const uint32_t grid = (NW + NXCD - 1) / NXCD * chunks * NXCD;
-    k_isa_scatter<<<grid, 256, 0, s>>>(pairs, N, wshift, chunks, NW, ISA);
+    const size_t chunks = ((size_t)N + SC_TILE - 1) / SC_TILE;
+    const size_t runs = (chunks + SC_SUPER - 1) / SC_SUPER;
+    const size_t grid = (runs + NXCD - 1) / NXCD * NXCD * SC_SUPER;
+    k_isa_scatter<<<(uint32_t)grid, 256, 0, s>>>(pairs, N, ISA);
 }
 void launch_regroup(const RegroupArgs &a, hipStream_t s) {
     const uint32_t ntiles = (a.m + RG_TILE - 1) / RG_TILE;
     if (!ntiles) return;
     k_regroup<RG_BLOCK, RG_ITEMS><<<ntiles, RG_BLOCK, 0, s>>>(a.K, a.V, a.pos, a.m, a.kshift, a.blk, a.SA, a.ISA,
-                                                             a.pairs, a.L,
-                                                             a.T, a.pos_out, a.grp_out, a.m_out, a.st_sum, a.st_max,
-                                                             a.counter, a.epoch, ntiles);
+                                                             a.pairs, a.L, a.T, a.pos_out, a.grp_out, a.m_out,
+                                                             a.st_sum, a.st_max, a.counter, a.epoch, ntiles);
 }
 void launch_round_keys(const uint32_t *pos, const uint32_t *grp, uint32_t m, const uint32_t *SA, const uint32_t *ISA,
                        uint32_t n, const uint16_t *blk, const BlockInfo *blocks, uint32_t round, uint64_t *K, uint32_t *V,
