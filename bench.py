@@ -40,7 +40,7 @@ KERNEL_BYTES = {
     "hist_T": 1.0,         # read T
     "byte_hist": 1.0,
     "hist_keys": 8.0,      # read key
-    "regroup": 18.0,       # read key 8 + value 4, write ISA 4, SA row byte 1 (+ compaction of the active rows)
+    "regroup": 18.0,       # read key 8 + value 4, write ISA 4, SA row byte 1 (+ the group lists: <= 4 per row)
     "round_keys": 32.0,    # read pos, grp, SA, ISA[i+h] (4 each), write key 8 + value 4, plus the active row's SA read
     "bwt": 2.0,            # read the row byte, write the BWT byte
     "lf_pass": 5.0,        # read L 1, write LF 4
@@ -49,7 +49,6 @@ KERNEL_BYTES = {
     "walk_emit": 2.0,      # one staged byte read and written per step
     "rank_jump": 24.0,     # read two links, write one
     "fill_blk": 2.0,
-    "to_groups": 10.0,     # read pos 4 + group head 4 per active row, write (head, size) 8 per group (<= 4 per row)
     "scan_groups": 8.0,    # read a group size, write its first-row offset
     "large_elems": 8.0,    # write pos + group head per row of a large group
     "seg_gather": 12.0,    # read the row's suffix 4 and ISA[i+h] 4, write the sort key 4

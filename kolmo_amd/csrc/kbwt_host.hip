@@ -53,7 +53,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     std::mutex mu;
     uint32_t epoch = 1;
-    Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, st_max, samples, stage, blk, counts, dbase,
+    Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, carry, samples, stage, blk, counts, dbase,
         tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es, ecnt, eoff, pair, tmax;
     uint32_t *counters = nullptr;           // 16 tile counters (zeroed once); [15]: the device epoch of graphs
     uint32_t *gcnt = nullptr;               // 4 group-list counters
@@ -363,14 +363,20 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     grow(w.SA, NN * 4);
     grow(w.ISA, NN * 4);
     grow(w.L, NN);
+    grow(w.pos[0], NN * 4);                    // rows of the large groups (SA position, group head)
+    grow(w.grp[0], NN * 4);
+    grow(w.carry, ((NN + RG_TILE - 1) / RG_TILE) * 4);
+    // group lists of the doubling rounds (groups have two rows or more), ping-pong between rounds
+    const size_t GMAX = NN / 2 + 2;
     for (int i = 0; i < 2; i++) {
-        grow(w.pos[i], NN * 4);
-        grow(w.grp[i], NN * 4);
+        grow(w.gh[i], GMAX * 4);
+        grow(w.gs[i], GMAX * 4);
+        grow(w.lh[i], GMAX * 4);
+        grow(w.ls[i], GMAX * 4);
     }
     const size_t ntiles_max = (NN + MIN_TILE - 1) / MIN_TILE + B;
     grow(w.status, ntiles_max * 256 * 8, true);
     grow(w.st_sum, ntiles_max * 8, true);
-    grow(w.st_max, ntiles_max * 8, true);
     grow(w.samples, (size_t)nsamp * 4);
     grow(w.counts, (size_t)B * MAXP * 256 * 4, true);
     grow(w.tot, (size_t)B * 256 * 4);
@@ -486,18 +492,19 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         g.m = N;
         g.kshift = 8;
         g.blk = blk;
+        g.blocks = d_bl;
         g.SA = nullptr;
         g.ISA = ISA;
         g.pairs = K[P0 & 1];       // the other key buffer
         g.L = L;
         g.T = nullptr;
-        g.pos_out = P<uint32_t>(w.pos[0]);
-        g.grp_out = P<uint32_t>(w.grp[0]);
-        g.m_out = w.m_out;
-        g.st_sum = P<uint64_t>(w.st_sum);
-        g.st_max = P<uint64_t>(w.st_max);
-        g.counter = w.counters + 1;
-        g.epoch = next_epoch(w);
+        g.carry = P<uint32_t>(w.carry);
+        g.sh = P<uint32_t>(w.gh[0]);
+        g.ss = P<uint32_t>(w.gs[0]);
+        g.lh = P<uint32_t>(w.lh[0]);
+        g.ls = P<uint32_t>(w.ls[0]);
+        g.cnt = w.gcnt;
+        launch_fill_u8(reinterpret_cast<uint8_t *>(w.gcnt), 0, 16, s);
         TIMED("regroup", N, launch_regroup(g, s));
         LAUNCHCHK();
         // ISA[suffix] = head: partition the pairs by suffix window inside each block (one keys-only radix pass),
@@ -509,19 +516,9 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, ISA, s));
     }
     LAUNCHCHK();
-    HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    uint32_t m = *w.h_m;
 
     // 5. doubling rounds over the groups of two or more (SA ranges). Small groups sort inside one workgroup each
-    // (k_seg_round); groups of more than SEG_TILE rows take the global path (round keys, radix passes, regroup).
-    const size_t GMAX = NN / 2 + 2;   // groups have two rows or more
-    for (int i = 0; i < 2; i++) {
-        grow(w.gh[i], GMAX * 4);
-        grow(w.gs[i], GMAX * 4);
-        grow(w.lh[i], GMAX * 4);
-        grow(w.ls[i], GMAX * 4);
-    }
+    // (k_seg_round); groups of more than SEG_GROUP rows take the global path (round keys, radix passes, regroup).
     grow(w.go, GMAX * 4);
     grow(w.tstart, (NN / SEG_TILE + 2) * 4);
     grow(w.eh, (NN / SEG_TILE + 2) * SEG_TILE * 4);
@@ -532,7 +529,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     grow(w.sec, NN * 4);
     grow(w.sufl, NN * 4);
     grow(w.lo, GMAX * 8);
-    uint32_t *gcnt = w.gcnt;          // [small groups, large groups, small rows] of the list being built
+    uint32_t *gcnt = w.gcnt;          // [small groups, large groups] of the list being built
     const uint32_t rank_max = tune.seg_rank;
     std::vector<uint32_t> h_lh, h_ls, h_lo;
     auto read_lists = [&](int c, uint32_t &Gs, uint32_t &Gl) {
@@ -551,14 +548,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     auto zero_counts = [&] { launch_fill_u8(reinterpret_cast<uint8_t *>(gcnt), 0, 16, s); };
     int cur = 0, round = 0;
     uint32_t Gs = 0, Gl = 0;
-    zero_counts();
-    if (m) {
-        TIMED("to_groups", m,
-              launch_elems_to_groups(P<uint32_t>(w.pos[0]), P<uint32_t>(w.grp[0]), w.m_out, m, P<uint32_t>(w.gh[0]),
-                                     P<uint32_t>(w.gs[0]), P<uint32_t>(w.lh[0]), P<uint32_t>(w.ls[0]), gcnt, s));
-        LAUNCHCHK();
-    }
-    read_lists(0, Gs, Gl);
+    read_lists(0, Gs, Gl);   // the lists the first regroup wrote
     const Tiling flat{nullptr, d_bl, 1};
     while (Gs + Gl > 0) {
         round++;
@@ -685,18 +675,14 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
             g.ISA = ISA;
             g.L = L;
             g.T = dT;
-            g.pos_out = P<uint32_t>(w.pos[1]);
-            g.grp_out = P<uint32_t>(w.grp[1]);
-            g.m_out = w.m_out;
-            g.st_sum = P<uint64_t>(w.st_sum);
-            g.st_max = P<uint64_t>(w.st_max);
-            g.counter = w.counters + 1;
-            g.epoch = next_epoch(w);
-                TIMED("regroup", ml, launch_regroup(g, s));
-            TIMED("to_groups", ml,
-                  launch_elems_to_groups(P<uint32_t>(w.pos[1]), P<uint32_t>(w.grp[1]), w.m_out, ml,
-                                         P<uint32_t>(w.gh[cur ^ 1]), P<uint32_t>(w.gs[cur ^ 1]),
-                                         P<uint32_t>(w.lh[cur ^ 1]), P<uint32_t>(w.ls[cur ^ 1]), gcnt, s));
+            g.blocks = d_bl;
+            g.carry = P<uint32_t>(w.carry);
+            g.sh = P<uint32_t>(w.gh[cur ^ 1]);
+            g.ss = P<uint32_t>(w.gs[cur ^ 1]);
+            g.lh = P<uint32_t>(w.lh[cur ^ 1]);
+            g.ls = P<uint32_t>(w.ls[cur ^ 1]);
+            g.cnt = gcnt;
+            TIMED("regroup", ml, launch_regroup(g, s));
             LAUNCHCHK();
         }
         st.passes[round] = (int32_t)sh.P;
@@ -1365,7 +1351,7 @@ void kbwt_release(void) {
     Workspace &w = g_ws[dev];
     std::lock_guard<std::mutex> lk(w.mu);
     Buf *bufs[] = {&w.T,       &w.U,      &w.K0,    &w.K1,     &w.V0,     &w.V1,     &w.SA,     &w.ISA,    &w.L,
-                   &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.st_max, &w.samples,
+                   &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.carry, &w.samples,
                    &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
                    &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
                    &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,  &w.sufl,     &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax};

@@ -82,13 +82,14 @@ struct RegroupArgs {
     const uint32_t *V, *pos;
     uint32_t m, kshift;
     const uint16_t *blk;          // block of every packed index, or nullptr for one block
+    const BlockInfo *blocks;      // with blk: the blocks (their first rows bound the carry search)
     uint32_t *SA, *ISA;
     uint64_t *pairs;              // when set: (suffix << 32 | head) per row instead of the ISA writes
     uint8_t *L;
     const uint8_t *T;
-    uint32_t *pos_out, *grp_out, *m_out;
-    uint64_t *st_sum, *st_max;
-    uint32_t *counter, epoch;
+    uint32_t *carry;              // one word per RG_TILE rows: the head of the group holding the tile's first row
+    uint32_t *sh, *ss, *lh, *ls;  // the next round's small and large group lists (head, size)
+    uint32_t *cnt;                // their counters: cnt[0] small, cnt[1] large
 };
 
 struct SegRoundArgs {
@@ -107,8 +108,6 @@ struct SegRoundArgs {
     uint32_t rank_max;            // tiles whose groups have at most rank_max rows rank them directly
 };
 
-void launch_elems_to_groups(const uint32_t *pos, const uint32_t *grp, const uint32_t *mp, uint32_t mmax,
-                            uint32_t *sh, uint32_t *ss, uint32_t *lh, uint32_t *ls, uint32_t *cnt, hipStream_t s);
 void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *tot, uint32_t *tstart,
                         uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s);
 void launch_compact_emits(const uint32_t *eh, const uint32_t *es, const uint32_t *ecnt, const uint32_t *eoff,

@@ -12,8 +12,11 @@
 //      symbol codes above the BWT byte T[i-1]. A tile of BLOCK*ITEMS elements is ranked by wave64 ballot multisplit,
 //      its digit offsets come from a decoupled lookback over earlier tiles of its block, and it leaves through LDS
 //      so that equal digits go out as contiguous runs.
-//   4. k_regroup: group heads (rank = SA position of the group's first row) by a max-scan, ISA[suffix] = head, the
-//      row's BWT byte, and compaction of the rows still in groups of two or more, with decoupled lookback.
+//   4. k_regroup_carry + k_regroup: group heads (rank = SA position of the group's first row) by a max-scan inside
+//      each tile, whose carry-in (the head of the group holding the tile's first row) comes from a binary search over
+//      the sorted keys, so tiles need nothing from each other; ISA[suffix] = head (through pairs partitioned by ISA
+//      window in the first round), the row's BWT byte, and the groups of two or more appended to the next round's
+//      group lists.
 //   5. per doubling round with offset h: k_round_keys gathers (group head, ISA[i+h] + h, or the distance to the
 //      block's end when i+h lies past it) for the active rows, the same radix passes sort them (one flat range: the
 //      heads already keep blocks apart), and k_regroup splits the groups.
@@ -427,42 +430,61 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     }
 }
 
-// ------------------------------------------------------------------ regroup (group heads, ISA, compaction)
+// ------------------------------------------------------------------ regroup (group heads, ISA, group lists)
 
-// Sorted keys K[0..m) with values V (suffixes). Row a goes to SA position pos[a] (pos == nullptr: a). Rows a-1 and a
-// share a group when their keys agree above bit kshift (and, with blk, lie in the same block). Writes SA[pos[a]] =
-// V[a] (when SA != nullptr), ISA[V[a]] = head of its group (SA position of the group's first row; with pairs, the
-// pair (V[a] << 32 | head) goes to pairs[a] instead, for k_isa_scatter after a partition by suffix), the BWT byte of
-// row pos[a] into L (the key's low byte when T == nullptr, else T[V[a]-1]; the byte of a block's first suffix is
-// never used), and compacts the rows of groups of two or more into (pos_out, grp_out); the last tile writes their
-// count to *m_out.
+// Rows a-1 and a of sorted keys K share a group when their keys agree above bit kshift (and, with blk, lie in the same
+// block). The group of a row starts at the first row of its run of equal keys; its head is that row's SA position
+// (pos[row], or the row itself when pos == nullptr).
+//
+// Head of the group holding the first row of every regroup tile (tile t starts at row t * RG_TILE), by a binary
+// search over the keys: they are sorted within a block (and over the whole range when blk == nullptr). One thread per
+// tile, so the tiles of k_regroup need nothing from each other.
+__global__ __launch_bounds__(256) void k_regroup_carry(const uint64_t *__restrict__ K, const uint32_t *__restrict__ pos,
+                                                       uint32_t kshift, const uint16_t *__restrict__ blk,
+                                                       const BlockInfo *__restrict__ blocks, uint32_t ntiles,
+                                                       uint32_t *__restrict__ carry) {
+    const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= ntiles) return;
+    const uint32_t a0 = t * (uint32_t)RG_TILE;
+    const uint64_t v = K[a0] >> kshift;
+    uint32_t lo = blk ? blocks[blk[a0]].off : 0u, hi = a0;   // first row in [lo, a0] whose key is >= v
+    while (lo < hi) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if ((K[mid] >> kshift) < v) lo = mid + 1; else hi = mid;
+    }
+    carry[t] = pos ? pos[lo] : lo;
+}
+
+// Sorted keys K[0..m) with values V (suffixes). Row a goes to SA position pos[a] (pos == nullptr: a). Writes
+// SA[pos[a]] = V[a] (when SA != nullptr), ISA[V[a]] = head of its group (with pairs, the pair (V[a] << 32 | head)
+// goes to pairs[a] instead, for k_isa_scatter after a partition by suffix), the BWT byte of row pos[a] into L (the
+// key's low byte when T == nullptr, else T[V[a]-1]; the byte of a block's first suffix is never used), and appends
+// every group of two or more rows as (head, size) to the small list (sh, ss; size <= SEG_GROUP) or the large list
+// (lh, ls), one atomic per list and tile on cnt[0], cnt[1]; the lists' order is free. carry[tile]: the head of the
+// group holding the tile's first row (k_regroup_carry).
 template <int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ K, const uint32_t *__restrict__ V,
                                                    const uint32_t *__restrict__ pos, uint32_t m, uint32_t kshift,
                                                    const uint16_t *__restrict__ blk, uint32_t *__restrict__ SA,
                                                    uint32_t *__restrict__ ISA, uint64_t *__restrict__ pairs,
-                                                   uint8_t *__restrict__ L,
-                                                   const uint8_t *__restrict__ T, uint32_t *__restrict__ pos_out,
-                                                   uint32_t *__restrict__ grp_out, uint32_t *__restrict__ m_out,
-                                                   uint64_t *__restrict__ st_sum, uint64_t *__restrict__ st_max,
-                                                   uint32_t *__restrict__ counter, uint32_t epoch, uint32_t ntiles) {
+                                                   uint8_t *__restrict__ L, const uint8_t *__restrict__ T,
+                                                   const uint32_t *__restrict__ carry, uint32_t *__restrict__ sh,
+                                                   uint32_t *__restrict__ ss, uint32_t *__restrict__ lh,
+                                                   uint32_t *__restrict__ ls, uint32_t *__restrict__ cnt_out) {
     // Rows come from HBM through LDS (coalesced loads in row order) and each thread works on ITEMS consecutive rows,
-    // so one block-wide scan covers the tile. The pairs leave through LDS in row order; the compacted rows (a
-    // thread's are consecutive in the output) are written from registers, which keeps the LDS footprint at four
-    // workgroups per CU. The LDS row of thread t starts at
-    // t * (ITEMS + 1): the pad keeps a wave's lanes on distinct banks.
+    // so one block-wide scan covers the tile. The pairs leave through LDS in row order. The LDS row of thread t
+    // starts at t * (ITEMS + 1): the pad keeps a wave's lanes on distinct banks.
     constexpr int TILE = BLOCK * ITEMS, PITCH = ITEMS + 1;
     static_assert(ITEMS == 8, "a thread's BWT bytes leave as one 64-bit word");
+    static_assert(TILE == RG_TILE, "k_regroup_carry's tiles");
     __shared__ uint64_t s_k[BLOCK * PITCH];      // keys, later the (suffix, head) pairs
     __shared__ uint32_t s_v[BLOCK * PITCH];      // suffixes
     __shared__ uint16_t s_b[BLOCK * PITCH];       // block ids (batches)
     __shared__ uint32_t s_scratch[32];
     __shared__ uint32_t s_imax[BLOCK];
-    __shared__ uint32_t s_tile, s_ex_sum, s_ex_max;
+    __shared__ uint32_t s_base[2];
     const int tid = threadIdx.x;
-    if (tid == 0) s_tile = next_tile(counter, ntiles);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t tile = blockIdx.x;
     const size_t t0 = (size_t)tile * TILE;
     const uint32_t cnt = (uint32_t)(t0 + TILE <= m ? TILE : m - t0);
     auto at = [](uint32_t j) { return (j / ITEMS) * PITCH + j % ITEMS; };
@@ -479,6 +501,7 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
     const bool has_prev = t0 > 0, has_next = t0 + cnt < m;
     const uint64_t kprev_t = has_prev ? K[t0 - 1] >> kshift : 0, knext_t = has_next ? K[t0 + cnt] >> kshift : 0;
     const uint16_t bprev_t = has_prev && blk ? blk[t0 - 1] : 0, bnext_t = has_next && blk ? blk[t0 + cnt] : 0;
+    const uint32_t head_in = carry[tile];
     __syncthreads();
 
     const uint32_t r0 = (uint32_t)tid * ITEMS;   // this thread's first row in the tile
@@ -499,8 +522,8 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
         lbyte[k] = j < cnt ? (uint8_t)s_k[at(j)] : 0;
         lp[k] = j < cnt ? (pos ? pos[t0 + j] : (uint32_t)(t0 + j)) : 0u;
     }
-    uint32_t flags = 0, keeps = 0;               // bit k: row k starts a group / stays active
-    uint32_t nkeep = 0, tmax = 0;
+    uint32_t flags = 0, ends = 0;                // bit k: row k starts a group / ends a group of two or more
+    uint32_t tmax = 0;
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const uint32_t j = r0 + k;
@@ -509,47 +532,23 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
         const bool flag = !pv || kc[k] != kc[k + 1] || bk[k] != bk[k + 1];
         const bool flagn = !nx || kc[k + 2] != kc[k + 1] || bk[k + 2] != bk[k + 1];
         if (flag) { flags |= 1u << k; tmax = lp[k]; }   // SA positions increase along the rows
-        if (!(flag && flagn)) { keeps |= 1u << k; nkeep++; }
+        else if (flagn) ends |= 1u << k;
     }
     uint32_t ex_s, in_m, tot_s, tot_m;
-    block_scan_sum_max<BLOCK>(nkeep, tmax, ex_s, in_m, tot_s, tot_m, s_scratch);
+    block_scan_sum_max<BLOCK>(0u, tmax, ex_s, in_m, tot_s, tot_m, s_scratch);
     s_imax[tid] = in_m;
-    // wave 0 looks back over the compaction counts, wave 1 over the group heads
-    const int wave = tid / WAVE;
-    if (wave == 0) {
-        uint32_t ex = 0;
-        if (tile == 0) {
-            if (tid == 0) st_store(st_sum, st_pack(epoch, ST_INC, tot_s));
-        } else {
-            if (tid == 0) st_store(st_sum + tile, st_pack(epoch, ST_AGG, tot_s));
-            ex = lookback_wave<false>(st_sum, tile, epoch);
-            if (tid == 0) st_store(st_sum + tile, st_pack(epoch, ST_INC, ex + tot_s));
-        }
-        if (tid == 0) {
-            s_ex_sum = ex;
-            if (tile == ntiles - 1) *m_out = ex + tot_s;
-        }
-    } else if (wave == 1) {
-        uint32_t ex_m = 0;
-        if (tile == 0) {
-            if (tid == WAVE) st_store(st_max, st_pack(epoch, ST_INC, tot_m));
-        } else {
-            if (tid == WAVE) st_store(st_max + tile, st_pack(epoch, ST_AGG, tot_m));
-            ex_m = lookback_wave<true>(st_max, tile, epoch);
-            const uint32_t im = ex_m > tot_m ? ex_m : tot_m;
-            if (tid == WAVE) st_store(st_max + tile, st_pack(epoch, ST_INC, im));
-        }
-        if (tid == WAVE) s_ex_max = ex_m;
-    }
     __syncthreads();
-    // the head before this thread's first row: the tile's prefix and the threads before this one
-    uint32_t head = s_ex_max;
+    // the head before this thread's first row: the tile's carry and the threads before this one
+    uint32_t head = head_in;
     if (tid > 0 && s_imax[tid - 1] > head) head = s_imax[tid - 1];
-    uint32_t o = s_ex_sum + ex_s;                 // compacted rows before this thread's
+    uint32_t eh[ITEMS], es[ITEMS];
+    uint32_t ns = 0, nl = 0;
     uint64_t lbytes = 0;
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const uint32_t j = r0 + k;
+        eh[k] = 0;
+        es[k] = 0;
         if (j >= cnt) continue;
         if (flags & (1u << k)) head = lp[k] > head ? lp[k] : head;
         const uint32_t i = s_v[at(j)];
@@ -559,10 +558,10 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
         const uint8_t c = T ? (i ? T[i - 1] : (uint8_t)0) : lbyte[k];
         if (pos) L[lp[k]] = c;
         else lbytes |= (uint64_t)c << (8 * k);
-        if (keeps & (1u << k)) {   // a thread's kept rows are consecutive in the output
-            pos_out[o] = lp[k];
-            grp_out[o] = head;
-            o++;
+        if (ends & (1u << k)) {
+            eh[k] = head;
+            es[k] = lp[k] - head + 1;
+            if (es[k] > (uint32_t)SEG_GROUP) nl++; else ns++;
         }
     }
     if (!pos) {   // rows in SA order: the thread's ITEMS bytes of L are consecutive
@@ -571,9 +570,22 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
             for (int k = 0; k < ITEMS; k++)
                 if (r0 + k < cnt) L[t0 + r0 + k] = (uint8_t)(lbytes >> (8 * k));
     }
-    if (!pairs) return;
+    // the tile's groups into the lists: one atomic per list
+    uint32_t tot;
+    const uint32_t ex = block_excl_sum<BLOCK>(ns | (nl << 16), s_scratch, &tot);
+    if (tid == 0) {
+        s_base[0] = (tot & 0xFFFFu) ? atomicAdd(&cnt_out[0], tot & 0xFFFFu) : 0u;
+        s_base[1] = (tot >> 16) ? atomicAdd(&cnt_out[1], tot >> 16) : 0u;
+    }
     __syncthreads();
-    // coalesced stores of the pairs in row order
+    uint32_t at_s = s_base[0] + (ex & 0xFFFFu), at_l = s_base[1] + (ex >> 16);
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        if (es[k] > (uint32_t)SEG_GROUP) { lh[at_l] = eh[k]; ls[at_l] = es[k]; at_l++; }
+        else if (es[k]) { sh[at_s] = eh[k]; ss[at_s] = es[k]; at_s++; }
+    }
+    if (!pairs) return;
+    // coalesced stores of the pairs in row order (the barrier above ordered the pair writes before these reads)
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const uint32_t j = k * BLOCK + tid;
@@ -1043,9 +1055,10 @@ void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t *ISA, hipStr
 void launch_regroup(const RegroupArgs &a, hipStream_t s) {
     const uint32_t ntiles = (a.m + RG_TILE - 1) / RG_TILE;
     if (!ntiles) return;
+    k_regroup_carry<<<(ntiles + 255) / 256, 256, 0, s>>>(a.K, a.pos, a.kshift, a.blk, a.blocks, ntiles, a.carry);
     k_regroup<RG_BLOCK, RG_ITEMS><<<ntiles, RG_BLOCK, 0, s>>>(a.K, a.V, a.pos, a.m, a.kshift, a.blk, a.SA, a.ISA,
-                                                             a.pairs, a.L, a.T, a.pos_out, a.grp_out, a.m_out,
-                                                             a.st_sum, a.st_max, a.counter, a.epoch, ntiles);
+                                                             a.pairs, a.L, a.T, a.carry, a.sh, a.ss, a.lh, a.ls,
+                                                             a.cnt);
 }
 void launch_round_keys(const uint32_t *pos, const uint32_t *grp, uint32_t m, const uint32_t *SA, const uint32_t *ISA,
                        uint32_t n, const uint16_t *blk, const BlockInfo *blocks, uint32_t round, uint64_t *K, uint32_t *V,

@@ -7,11 +7,11 @@
 // the new ranks (ISA), the rows are rewritten in SA with their BWT bytes, and subgroups of two or more form the next
 // round's list. All keys of a round are gathered before any rank of the round changes.
 //
-// Small groups (at most SEG_TILE suffixes) never leave one workgroup: a tile owns the groups whose first element
+// Small groups (at most SEG_GROUP suffixes) never leave one workgroup: a tile owns the groups whose first element
 // offset (prefix sum of sizes over the list) lies in its SEG_TILE-wide range, gathers them into LDS and sorts them
-// there by (group, sec) with an LDS radix sort, so a round costs a few bytes per suffix instead of the global radix
-// passes. Larger groups take the global path (k_round_keys, radix passes, k_regroup) and come back here through
-// k_elems_to_groups.
+// there by (group, sec) with an LDS radix sort or direct ranking, so a round costs a few bytes per suffix instead of
+// the global radix passes. Larger groups take the global path (k_round_keys, radix passes, k_regroup), whose
+// regroup appends the subgroups to the next round's lists itself.
 #include "kbwt_device.h"
 #include "kbwt_internal.h"
 
@@ -25,61 +25,6 @@ __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t *go, uint32_t
         if (go[mid] < x) lo = mid + 1; else hi = mid;
     }
     return lo;
-}
-
-// Element rows of compacted active rows (pos_out, grp_out: SA position and group head of each, in SA order, groups
-// contiguous) to group descriptors: the last row of each group emits (head, size = pos - head + 1) into the small
-// list (size <= SEG_GROUP) or the large list. A workgroup takes tiles of E2G_TILE rows and makes one atomic append per
-// list and tile (appends to one counter from every wave serialise at the memory side); the lists' order is free.
-// *mp: rows.
-constexpr int E2G_ITEMS = 16, E2G_TILE = 256 * E2G_ITEMS;
-__global__ __launch_bounds__(256) void k_elems_to_groups(const uint32_t *__restrict__ pos, const uint32_t *__restrict__ grp,
-                                                         const uint32_t *__restrict__ mp, uint32_t *__restrict__ sh,
-                                                         uint32_t *__restrict__ ss, uint32_t *__restrict__ lh,
-                                                         uint32_t *__restrict__ ls, uint32_t *__restrict__ cnt) {
-    __shared__ uint32_t s_scratch[8];
-    __shared__ uint32_t s_base[2];
-    const uint32_t m = *mp;
-    const int tid = threadIdx.x;
-    for (size_t t0 = (size_t)blockIdx.x * E2G_TILE; t0 < m; t0 += (size_t)gridDim.x * E2G_TILE) {
-        const size_t b = t0 + (size_t)tid * E2G_ITEMS;   // blocked: a thread owns E2G_ITEMS consecutive rows
-        uint32_t head[E2G_ITEMS], size[E2G_ITEMS];
-        uint32_t kind = 0;                               // bit k: row k ends a group; bit 16 + k: that group is large
-        uint32_t ns = 0, nl = 0;
-#pragma unroll
-        for (int k = 0; k < E2G_ITEMS; k++) {
-            const size_t o = b + k;
-            head[k] = o < m ? grp[o] : 0u;
-            size[k] = o < m ? pos[o] - head[k] + 1 : 0u;
-        }
-        const uint32_t next = b + E2G_ITEMS < m ? grp[b + E2G_ITEMS] : 0xFFFFFFFFu;
-#pragma unroll
-        for (int k = 0; k < E2G_ITEMS; k++) {
-            const size_t o = b + k;
-            const uint32_t nh = k + 1 < E2G_ITEMS ? head[k + 1] : next;
-            const bool last = o < m && (o + 1 == m || nh != head[k]);
-            if (last) {
-                kind |= 1u << k;
-                if (size[k] > (uint32_t)SEG_GROUP) { kind |= 1u << (16 + k); nl++; } else ns++;
-            }
-        }
-        uint32_t tot;
-        const uint32_t ex = block_excl_sum<256>(ns | (nl << 16), s_scratch, &tot);
-        if (tid == 0) {
-            s_base[0] = (tot & 0xFFFFu) ? atomicAdd(&cnt[0], tot & 0xFFFFu) : 0u;
-            s_base[1] = (tot >> 16) ? atomicAdd(&cnt[1], tot >> 16) : 0u;
-        }
-        __syncthreads();
-        uint32_t at_s = s_base[0] + (ex & 0xFFFFu), at_l = s_base[1] + (ex >> 16);
-#pragma unroll
-        for (int k = 0; k < E2G_ITEMS; k++) {
-            if (kind & (1u << k)) {
-                if (kind & (1u << (16 + k))) { lh[at_l] = head[k]; ls[at_l] = size[k]; at_l++; }
-                else { sh[at_s] = head[k]; ss[at_s] = size[k]; at_s++; }
-            }
-        }
-        __syncthreads();   // s_base is rewritten by the next tile
-    }
 }
 
 // Exclusive prefix sum of the group sizes: go[g] = sizes before g; the last tile writes the total to *tot. Also the
@@ -507,12 +452,6 @@ __global__ __launch_bounds__(256) void k_compact_emits(const uint32_t *__restric
 
 // ------------------------------------------------------------------ launch wrappers
 
-void launch_elems_to_groups(const uint32_t *pos, const uint32_t *grp, const uint32_t *mp, uint32_t mmax,
-                            uint32_t *sh, uint32_t *ss, uint32_t *lh, uint32_t *ls, uint32_t *cnt, hipStream_t s) {
-    uint32_t grid = (uint32_t)(((size_t)mmax + E2G_TILE - 1) / E2G_TILE);
-    if (grid > 2048) grid = 2048;
-    if (grid) k_elems_to_groups<<<grid, 256, 0, s>>>(pos, grp, mp, sh, ss, lh, ls, cnt);
-}
 void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *tot, uint32_t *tstart,
                         uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s) {
     const uint32_t ntiles = (G + 2047) / 2048;
