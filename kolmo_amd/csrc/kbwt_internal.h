@@ -13,7 +13,7 @@ constexpr int MAXP = 8;                      // 8-bit digits of a 64-bit key
 constexpr int RG_BLOCK = 256, RG_ITEMS = 8, RG_TILE = RG_BLOCK * RG_ITEMS;
 constexpr int LF_BLOCK = 256, LF_ITEMS = 32, LF_TILE = LF_BLOCK * LF_ITEMS;
 constexpr int HIST_TILE = 4096;              // bytes per tile of the histogram kernels
-constexpr int HIST_GRID = 512;               // workgroups of the histogram kernels
+constexpr int HIST_GRID = 512;               // workgroups of k_hist_keys
 constexpr int WALK_TILE = 256;               // splitters per workgroup of the inverse walks
 constexpr uint32_t MIN_TILE = 2048;          // smallest tile of any tiled kernel (sizes the lookback arrays)
 constexpr int SEG_TILE = 1024;               // doubling rounds: rows of the group list per tile (by first row)

@@ -990,8 +990,11 @@ __global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const u
 
 // ------------------------------------------------------------------ launch wrappers
 
+// The text histograms run four workgroups per CU (their LDS allows it): each one's LDS atomics wait long, and more
+// waves per SIMD hide that (k_hist_T: 0.131 -> 0.103 ms on a 32 MiB block against 512 workgroups).
+constexpr uint32_t TEXT_HIST_GRID = 1024;
 void launch_byte_hist(const uint8_t *T, Tiling tl, uint32_t ntiles, uint32_t *counts, hipStream_t s) {
-    const uint32_t grid = ntiles < (uint32_t)HIST_GRID ? ntiles : (uint32_t)HIST_GRID;
+    const uint32_t grid = ntiles < TEXT_HIST_GRID ? ntiles : TEXT_HIST_GRID;
     if (grid) k_byte_hist<<<grid, 256, 0, s>>>(T, tl, ntiles, counts);
 }
 void launch_scan_digits(uint32_t *counts, uint32_t nsets, uint32_t sets_per_block, const BlockInfo *blocks,
@@ -1000,7 +1003,7 @@ void launch_scan_digits(uint32_t *counts, uint32_t nsets, uint32_t sets_per_bloc
 }
 void launch_hist_T(const uint8_t *T, Tiling tl, uint32_t ntiles, const uint8_t *codes, uint32_t P, uint32_t *counts,
                    hipStream_t s) {
-    const uint32_t grid = ntiles < (uint32_t)HIST_GRID ? ntiles : (uint32_t)HIST_GRID;
+    const uint32_t grid = ntiles < TEXT_HIST_GRID ? ntiles : TEXT_HIST_GRID;
     if (grid) k_hist_T<<<grid, 256, 0, s>>>(T, tl, ntiles, codes, P, counts);
 }
 void launch_hist_keys(const uint64_t *K, uint32_t m, const Shifts &sh, uint32_t *counts, uint32_t grid,
