@@ -734,10 +734,11 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     grow(w.pos[1], ((size_t)S + 1) * 4);       // continuation rows
     grow(w.K0, ((size_t)S + 1) * 8);           // links (ping)
     grow(w.K1, ((size_t)S + 1) * 8);           // links (pong)
-    // step-major stage: cap rows of sstride bytes, at most STAGE_MAX bytes in all
+    // chunk-major stage: cap / 16 rows of 16-byte chunks, one per splitter (sstride >= S of them), at most STAGE_MAX
+    // bytes in all
     const size_t sstride = ((size_t)S + 63) & ~(size_t)63;
-    if ((size_t)cap * sstride > STAGE_MAX) cap = (uint32_t)std::max<size_t>(2, STAGE_MAX / sstride);
-    cap = std::max(2u, cap & ~1u);             // the counting walk stages two steps at a time
+    if ((size_t)cap * sstride > STAGE_MAX) cap = (uint32_t)std::max<size_t>(16, STAGE_MAX / sstride);
+    cap = std::max(16u, cap & ~15u);           // the counting walk stages whole 16-step chunks
     grow(w.pair, NN * 8);                      // two LF steps per row
     grow(w.stage, (size_t)cap * sstride);
     const size_t ntiles_max = (NN + LF_TILE - 1) / LF_TILE + B;

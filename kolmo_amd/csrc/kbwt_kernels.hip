@@ -808,10 +808,11 @@ __global__ __launch_bounds__(LF_BLOCK) void k_lf_pairs(const uint32_t *__restric
 
 // Splitters of block b: local j < nreg is the row j*D, j == nreg the row P (only when P % D != 0); global id
 // spoff + j. From each, walk LF to the next splitter row of the block; link = (steps << 32) | next splitter, len =
-// steps. The bytes met on the way (the text backwards from the splitter's suffix) are staged step-major,
-// stage[t * sstride + id] for the first cap steps t (cap even), so that the lanes of a wave, which walk consecutive
-// splitters in step, store consecutive bytes; cont keeps the local row of step cap of a longer sublist. The marker
-// row P ends the list: link = (0, S). Workgroups take WALK_TILE splitters of one block.
+// steps. The bytes met on the way (the text backwards from the splitter's suffix) are staged for the first cap steps
+// (cap a multiple of 16) in 16-byte chunks, chunk-major: steps 16c..16c+15 of splitter id at stage[c * 16 * sstride
+// + 16 * id]. The lanes of a wave walk consecutive splitters in step, so they store a chunk row together, 16 bytes
+// each; cont keeps the local row of step cap of a longer sublist. The marker row P ends the list: link = (0, S).
+// Workgroups take WALK_TILE splitters of one block.
 //
 // The walk is a chain of dependent loads, so its time is the longest sublist times the load latency. It moves two
 // steps per load through the pair table (one when the first step lands on a splitter), and the next load is issued
@@ -838,12 +839,24 @@ __global__ __launch_bounds__(WALK_TILE) void k_walk_count(const uint64_t *__rest
     if (r0 == bi.P) { link[id] = S; len[id] = 0; return; }
     uint32_t steps = 0, r = r0;
     uint64_t e = pair[bi.off + r0 - (r0 < bi.P ? 0u : 1u)];
-    uint8_t *st = stage + id;
+    uint8_t *st = stage + (size_t)id * 16;
+    const size_t cstride = sstride * 16;
+    uint64_t lo = 0, hi = 0;                                   // the chunk being filled: steps (steps & ~15) + 0..15
+    auto put = [&](uint32_t t, uint64_t bytes) {               // bytes of step t (t even) and on, within the chunk
+        const uint64_t v = bytes << (8 * (t & 7));
+        if (t & 8) hi |= v; else lo |= v;
+    };
+    auto store = [&](uint32_t c) {
+        *reinterpret_cast<uint4 *>(st + (size_t)c * cstride) =
+            make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+        lo = 0;
+        hi = 0;
+    };
     for (;;) {
         const uint32_t x = (uint32_t)e;
         const uint8_t b1 = (uint8_t)(e >> 32);
         if ((e >> 40) & 1u) {                                  // the first step lands on a splitter
-            if (steps < cap) st[(size_t)steps * sstride] = b1;
+            if (steps < cap) put(steps, b1);
             steps++;
             r = x - bi.roff;
             break;
@@ -855,13 +868,16 @@ __global__ __launch_bounds__(WALK_TILE) void k_walk_count(const uint64_t *__rest
             if (steps + 2 == cap) cont[id] = rn;
             en = pair[bi.off + rn - (rn < bi.P ? 0u : 1u)];
         }
-        if (steps < cap) st[(size_t)steps * sstride] = b1;
-        if (steps + 1 < cap) st[(size_t)(steps + 1) * sstride] = row_byte(s_c, x);
+        if (steps < cap) {                                     // cap is a multiple of 16: both steps fit
+            put(steps, (uint64_t)b1 | ((uint64_t)row_byte(s_c, x) << 8));
+            if ((steps & 15) == 14) store(steps >> 4);
+        }
         steps += 2;
         r = rn;
         if (!more) break;
         e = en;
     }
+    if ((steps & 15) != 0 && steps <= cap) store((steps - 1) >> 4);   // the last, partial chunk
     const uint32_t nxt = r == bi.P ? sidP : bi.spoff + (r >> dshift);
     link[id] = ((uint64_t)steps << 32) | nxt;
     len[id] = steps;
@@ -899,8 +915,9 @@ __global__ __launch_bounds__(256) void k_check_lists(const uint64_t *__restrict_
 
 // Write every sublist: its splitter's suffix starts at block position pos = link >> 32 (after ranking), and step t of
 // the walk is the byte at pos-1-t. The staged steps go through LDS, EMIT_ROWS steps of the workgroup's WALK_TILE
-// sublists at a time: rows of the step-major stage are read whole, and each wave then writes the bytes of one
-// sublist at a time as one contiguous run. A sublist longer than cap walks on from cont.
+// sublists at a time: each lane reads its sublist's 16-byte chunks of those steps (only chunks it staged), and each
+// wave then writes the bytes of one sublist at a time as one contiguous run. A sublist longer than cap walks on from
+// cont.
 constexpr int EMIT_ROWS = 64, EMIT_PITCH = WALK_TILE + 4;   // pitch: a wave's lanes read one column without conflicts
 __global__ __launch_bounds__(WALK_TILE) void k_walk_emit(const uint32_t *__restrict__ lf, Tiling tl, uint32_t dshift,
                                                          uint32_t S, const uint64_t *__restrict__ link,
@@ -941,14 +958,19 @@ __global__ __launch_bounds__(WALK_TILE) void k_walk_emit(const uint32_t *__restr
     for (uint32_t t0 = 0; t0 < rows; t0 += EMIT_ROWS) {
         const uint32_t nr = rows - t0 < (uint32_t)EMIT_ROWS ? rows - t0 : (uint32_t)EMIT_ROWS;
         if ((uint32_t)tid < cnt) {
-            const uint8_t *src = stage + (size_t)t0 * sstride + id0 + tid;
-            for (uint32_t t = 0; t < nr; t += 16) {      // 16 loads in flight per lane
-                uint8_t v[16];
+            // chunks t0/16 .. t0/16 + EMIT_ROWS/16 - 1 of this lane's sublist, all loads in flight together
+            const uint8_t *src = stage + (size_t)(t0 >> 4) * 16 * sstride + (size_t)(id0 + tid) * 16;
+            uint4 v[EMIT_ROWS / 16];
 #pragma unroll
-                for (int u = 0; u < 16; u++) v[u] = t + u < nr && t0 + t + u < staged ? src[(size_t)(t + u) * sstride] : 0;
+            for (int q = 0; q < EMIT_ROWS / 16; q++)
+                v[q] = t0 + 16 * q < staged ? *reinterpret_cast<const uint4 *>(src + (size_t)q * 16 * sstride)
+                                            : make_uint4(0, 0, 0, 0);
+#pragma unroll
+            for (int q = 0; q < EMIT_ROWS / 16; q++) {
+                const uint32_t wd[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
 #pragma unroll
                 for (int u = 0; u < 16; u++)
-                    if (t + u < nr) s_rows[(t + u) * EMIT_PITCH + tid] = v[u];
+                    if (16 * q + u < (int)nr) s_rows[(16 * q + u) * EMIT_PITCH + tid] = (uint8_t)(wd[u >> 2] >> (8 * (u & 3)));
             }
         }
         __syncthreads();
