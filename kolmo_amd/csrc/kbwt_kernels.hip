@@ -27,6 +27,8 @@
 //   3. k_walk_count: from every splitter row (row % D == 0, and P) follow LF to the next splitter, recording the
 //      sublist length and staging its first bytes; k_rank_jump: synchronous pointer jumping gives each splitter's
 //      text position; k_walk_emit: each sublist's bytes land at their positions.
+#include <type_traits>
+
 #include "kbwt_device.h"
 #include "kbwt_internal.h"
 
@@ -401,33 +403,41 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
         uint32_t d = digit(key[k]);
         lidx[k] = s_tstart[d] + s_wcnt[wave][d] + rk[k];
     }
+    // A full tile (every tile but a block's last) takes the same steps without a check per item: straight-line code
+    // lets the compiler give every store its own registers, where the checked form reused one set and waited for
+    // each store to finish before the next.
+    auto scatter = [&](auto full_tile) {
+        constexpr bool FULL = decltype(full_tile)::value;
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++)
-        if (item(k) < cnt) s_buf[lidx[k]] = key[k];
-    __syncthreads();
-    uint32_t dg[ITEMS];
+        for (int k = 0; k < ITEMS; k++)
+            if (FULL || item(k) < cnt) s_buf[lidx[k]] = key[k];
+        __syncthreads();
+        uint32_t dg[ITEMS];
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        uint32_t j = tid + k * BLOCK;
-        dg[k] = 0;
-        if (j < cnt) {
-            uint64_t kk = s_buf[j];
-            dg[k] = digit(kk);
-            kout[s_gbase[dg[k]] + j] = kk;
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t j = tid + k * BLOCK;
+            dg[k] = 0;
+            if (FULL || j < cnt) {
+                const uint64_t kk = s_buf[j];
+                dg[k] = digit(kk);
+                kout[s_gbase[dg[k]] + j] = kk;
+            }
         }
-    }
-    if (!VALS) return;
-    __syncthreads();
-    uint32_t *s_v = reinterpret_cast<uint32_t *>(s_buf);
+        if (!VALS) return;
+        __syncthreads();
+        uint32_t *s_v = reinterpret_cast<uint32_t *>(s_buf);
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++)
-        if (item(k) < cnt) s_v[lidx[k]] = val[k];
-    __syncthreads();
+        for (int k = 0; k < ITEMS; k++)
+            if (FULL || item(k) < cnt) s_v[lidx[k]] = val[k];
+        __syncthreads();
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        uint32_t j = tid + k * BLOCK;
-        if (j < cnt) vout[s_gbase[dg[k]] + j] = s_v[j];
-    }
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t j = tid + k * BLOCK;
+            if (FULL || j < cnt) vout[s_gbase[dg[k]] + j] = s_v[j];
+        }
+    };
+    if (cnt == (uint32_t)TILE) scatter(std::true_type{});
+    else scatter(std::false_type{});
 }
 
 // ------------------------------------------------------------------ regroup (group heads, ISA, group lists)
@@ -788,22 +798,40 @@ __global__ __launch_bounds__(LF_BLOCK) void k_lf_pairs(const uint32_t *__restric
     const uint32_t dmask = (1u << dshift) - 1u;
     const size_t beg = (size_t)bi.off + (size_t)tb * LF_TILE;
     const size_t end = (size_t)bi.off + bi.n;
-    uint32_t v[LF_ITEMS];
+    // every row's LF, then every second lookup and byte, then every pair: each phase's loads are in flight together
+    // (the stored row of any landing row lies in the block, so a full tile gathers without checks)
+    auto run = [&](auto full_tile) {
+        constexpr bool FULL = decltype(full_tile)::value;
+        uint32_t v[LF_ITEMS], x[LF_ITEMS];
+        uint8_t c[LF_ITEMS];
 #pragma unroll
-    for (int k = 0; k < LF_ITEMS; k++) {
-        const size_t q = beg + (size_t)k * LF_BLOCK + threadIdx.x;
-        v[k] = q < end ? lf[q] : 0u;
-    }
-#pragma unroll
-    for (int k = 0; k < LF_ITEMS; k++) {
-        const size_t q = beg + (size_t)k * LF_BLOCK + threadIdx.x;
-        if (q < end) {
-            const uint32_t row = v[k] - bi.roff;
-            const bool stop = (row & dmask) == 0 || row == bi.P;
-            const uint32_t x = stop ? v[k] : lf[bi.off + row - (row < bi.P ? 0u : 1u)];
-            pair[q] = ((uint64_t)stop << 40) | ((uint64_t)L[q] << 32) | x;
+        for (int k = 0; k < LF_ITEMS; k++) {
+            const size_t q = beg + (size_t)k * LF_BLOCK + threadIdx.x;
+            v[k] = FULL || q < end ? lf[q] : 0u;
         }
-    }
+#pragma unroll
+        for (int k = 0; k < LF_ITEMS; k++) {
+            const size_t q = beg + (size_t)k * LF_BLOCK + threadIdx.x;
+            const uint32_t row = v[k] - bi.roff;
+            x[k] = 0;
+            c[k] = 0;
+            if (FULL || q < end) {
+                x[k] = lf[bi.off + row - (row < bi.P ? 0u : 1u)];
+                c[k] = L[q];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < LF_ITEMS; k++) {
+            const size_t q = beg + (size_t)k * LF_BLOCK + threadIdx.x;
+            if (FULL || q < end) {
+                const uint32_t row = v[k] - bi.roff;
+                const bool stop = (row & dmask) == 0 || row == bi.P;
+                pair[q] = ((uint64_t)stop << 40) | ((uint64_t)c[k] << 32) | (stop ? v[k] : x[k]);
+            }
+        }
+    };
+    if (beg + LF_TILE <= end) run(std::true_type{});
+    else run(std::false_type{});
 }
 
 // Splitters of block b: local j < nreg is the row j*D, j == nreg the row P (only when P % D != 0); global id

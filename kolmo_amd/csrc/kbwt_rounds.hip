@@ -96,12 +96,6 @@ __global__ __launch_bounds__(256) void k_large_elems(const uint32_t *__restrict_
     }
 }
 
-// Sort key of suffix i at offset h inside its block [.., end).
-__device__ __forceinline__ uint32_t sec_of(const uint32_t *ISA, uint32_t i, uint32_t h, uint32_t end) {
-    const uint64_t j = (uint64_t)i + h;
-    return j < end ? ISA[j] + h : end - 1u - i;
-}
-
 // One tile of small groups. MODE 0 gathers every row's key and records in tmax the rows of the tile's largest group
 // that splits (whose keys are not all equal; 0 when none does). The split sorts each group by its key in LDS,
 // rewrites SA, ISA and the BWT bytes of its rows, and writes the subgroups of two or more into the tile's own area of
@@ -195,13 +189,22 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
             const uint32_t e = tid + k * 256;
             gi[k] = e < cnt ? SA[s_gh[ga[k]] + (e - s_go[ga[k]])] : 0u;
         }
+        // the key of suffix i at offset h inside its block [.., end): ISA[i + h] + h, or end - 1 - i past the end.
+        // Every key's ISA load is in flight together: the index is clamped into the block (and row 0 past the tile's
+        // rows) so that no load depends on a branch, and the past-the-end case is chosen afterwards
+        uint32_t raw[ITEMS];
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             const uint32_t e = tid + k * 256;
-            if (e < cnt) {
-                gv[k] = sec_of(ISA, gi[k], s_hoff[ga[k]], s_gend[ga[k]]);
-                if (e == s_go[ga[k]]) s_first[ga[k]] = gv[k];
-            }
+            const uint64_t j = (uint64_t)gi[k] + s_hoff[ga[k]];
+            raw[k] = ISA[e < cnt ? (j < s_gend[ga[k]] ? (uint32_t)j : s_gend[ga[k]] - 1u) : 0u];
+        }
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t e = tid + k * 256;
+            const uint32_t h = s_hoff[ga[k]], end = s_gend[ga[k]];
+            gv[k] = (uint64_t)gi[k] + h < end ? raw[k] + h : end - 1u - gi[k];
+            if (e < cnt && e == s_go[ga[k]]) s_first[ga[k]] = gv[k];
         }
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
