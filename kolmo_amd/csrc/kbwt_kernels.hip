@@ -498,13 +498,31 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
     const size_t t0 = (size_t)tile * TILE;
     const uint32_t cnt = (uint32_t)(t0 + TILE <= m ? TILE : m - t0);
     auto at = [](uint32_t j) { return (j / ITEMS) * PITCH + j % ITEMS; };
+    {   // all of the tile's loads in flight before the first LDS write (a full tile loads without checks)
+        uint64_t kk[ITEMS];
+        uint32_t vv[ITEMS];
+        uint16_t bb[ITEMS];
+        auto load = [&](auto full_tile) {
+            constexpr bool FULL = decltype(full_tile)::value;
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        const uint32_t j = k * BLOCK + tid;
-        if (j < cnt) {
-            s_k[at(j)] = K[t0 + j];
-            s_v[at(j)] = V[t0 + j];
-            if (blk) s_b[at(j)] = blk[t0 + j];
+            for (int k = 0; k < ITEMS; k++) {
+                const uint32_t j = k * BLOCK + tid;
+                const size_t a = FULL || j < cnt ? t0 + j : t0;   // a row of the tile either way
+                kk[k] = K[a];
+                vv[k] = V[a];
+                bb[k] = blk ? blk[a] : (uint16_t)0;
+            }
+        };
+        if (cnt == (uint32_t)TILE) load(std::true_type{});
+        else load(std::false_type{});
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t j = k * BLOCK + tid;
+            if (j < cnt) {
+                s_k[at(j)] = kk[k];
+                s_v[at(j)] = vv[k];
+                if (blk) s_b[at(j)] = bb[k];
+            }
         }
     }
     // the rows on either side of the tile
@@ -530,7 +548,8 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
     for (int k = 0; k < ITEMS; k++) {
         const uint32_t j = r0 + k;
         lbyte[k] = j < cnt ? (uint8_t)s_k[at(j)] : 0;
-        lp[k] = j < cnt ? (pos ? pos[t0 + j] : (uint32_t)(t0 + j)) : 0u;
+        const uint32_t pj = pos ? pos[t0 + (j < cnt ? j : cnt - 1)] : (uint32_t)(t0 + j);   // no branch on the load
+        lp[k] = j < cnt ? pj : 0u;
     }
     uint32_t flags = 0, ends = 0;                // bit k: row k starts a group / ends a group of two or more
     uint32_t tmax = 0;
@@ -672,18 +691,50 @@ __global__ __launch_bounds__(256) void k_round_keys(const uint32_t *__restrict__
 // ------------------------------------------------------------------ BWT emission
 
 // U of every block in libsais' convention from the row bytes L[p] = T[SA[p]-1]: U[0] = T[n-1], row p goes to U[p+1]
-// before the primary row p0 (the rank of the block's first suffix) and to U[p] after it. Four bytes per thread.
+// before the primary row p0 (the rank of the block's first suffix) and to U[p] after it. 16 bytes per thread: when
+// they lie in one block and U + o0 is 16-byte aligned they leave as one 16-byte store built from L[o0-1 .. o0+15],
+// otherwise byte by byte.
 __global__ __launch_bounds__(256) void k_bwt(const uint8_t *__restrict__ T, const uint8_t *__restrict__ L,
                                              const uint32_t *__restrict__ ISA, const uint16_t *__restrict__ blk,
                                              const BlockInfo *__restrict__ blocks, uint8_t *__restrict__ U,
                                              uint32_t N) {
-    size_t o0 = ((size_t)blockIdx.x * 256 + threadIdx.x) * 4;
+    const size_t o0 = ((size_t)blockIdx.x * 256 + threadIdx.x) * 16;
+    if (o0 >= N) return;
+    auto info = [&](size_t o, uint32_t &off, uint32_t &n) {
+        off = 0;
+        n = N;
+        if (blk) {
+            const BlockInfo bi = blocks[blk[o]];
+            off = bi.off;
+            n = bi.n;
+        }
+    };
+    const bool whole = o0 + 16 <= N && (reinterpret_cast<uintptr_t>(U + o0) & 15u) == 0 &&
+                       (!blk || blk[o0] == blk[o0 + 15]);
+    if (whole) {
+        uint32_t off, n;
+        info(o0, off, n);
+        const uint32_t p0 = ISA[off] - off;
+        const uint4 cur = *reinterpret_cast<const uint4 *>(L + o0);   // L comes from hipMalloc: o0 is 16-aligned
+        const uint32_t w[4] = {cur.x, cur.y, cur.z, cur.w};
+        uint8_t lb[17];
+        lb[0] = o0 > 0 ? L[o0 - 1] : (uint8_t)0;
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        size_t o = o0 + k;
-        if (o >= N) break;
-        uint32_t off = 0, n = N;
-        if (blk) { const BlockInfo bi = blocks[blk[o]]; off = bi.off; n = bi.n; }
+        for (int i = 0; i < 16; i++) lb[i + 1] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
+        uint32_t ow[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const size_t lo = o0 + i - off;
+            const uint8_t c = lo == 0 ? T[off + n - 1] : (lo <= p0 ? lb[i] : lb[i + 1]);
+            ow[i >> 2] |= (uint32_t)c << (8 * (i & 3));
+        }
+        *reinterpret_cast<uint4 *>(U + o0) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
+        return;
+    }
+    for (int i = 0; i < 16 && o0 + i < N; i++) {
+        const size_t o = o0 + i;
+        uint32_t off, n;
+        info(o, off, n);
         const uint32_t p0 = ISA[off] - off;
         const size_t lo = o - off;   // output byte lo of the block comes from row lo-1 (lo <= p0) or row lo (lo > p0)
         U[o] = lo == 0 ? T[off + n - 1] : (lo <= p0 ? L[o - 1] : L[o]);
@@ -769,11 +820,16 @@ __global__ __launch_bounds__(BLOCK) void k_lf_pass(const uint8_t *__restrict__ L
         s_base[tid] = cbase[(size_t)tr.b * 256 + tid] + excl;
     }
     __syncthreads();
+    auto store = [&](auto full_tile) {   // a full tile stores without a check per item (no wait between the stores)
+        constexpr bool FULL = decltype(full_tile)::value;
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
-        if (local < cnt) lf[tr.beg + local] = s_base[dg[k]] + s_wcnt[wave][dg[k]] + rk[k];
-    }
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t local = wave * WAVE * ITEMS + k * WAVE + lane;
+            if (FULL || local < cnt) lf[tr.beg + local] = s_base[dg[k]] + s_wcnt[wave][dg[k]] + rk[k];
+        }
+    };
+    if (cnt == (uint32_t)TILE) store(std::true_type{});
+    else store(std::false_type{});
 }
 
 // Byte of a row from its LF value (a global row): the c with C[c] <= v < C[c+1] (s_c holds the block's C[0..255]).
@@ -1122,7 +1178,7 @@ void launch_round_keys(const uint32_t *pos, const uint32_t *grp, uint32_t m, con
 }
 void launch_bwt(const uint8_t *T, const uint8_t *L, const uint32_t *ISA, const uint16_t *blk, const BlockInfo *blocks,
                 uint8_t *U, uint32_t N, hipStream_t s) {
-    if (N) k_bwt<<<(N + 1023) / 1024, 256, 0, s>>>(T, L, ISA, blk, blocks, U, N);
+    if (N) k_bwt<<<(uint32_t)(((size_t)N + 4095) / 4096), 256, 0, s>>>(T, L, ISA, blk, blocks, U, N);
 }
 void launch_samples(const uint32_t *ISA, const BlockInfo *blocks, uint32_t B, uint32_t count, int32_t *I,
                     hipStream_t s) {
