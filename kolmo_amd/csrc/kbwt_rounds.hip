@@ -187,7 +187,9 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             const uint32_t e = tid + k * 256;
-            gi[k] = e < cnt ? SA[s_gh[ga[k]] + (e - s_go[ga[k]])] : 0u;
+            const uint32_t r = s_gh[ga[k]] + (e - s_go[ga[k]]);   // an SA row of the tile's groups below cnt
+            const uint32_t a = SA[e < cnt ? r : s_gh[0]];
+            gi[k] = e < cnt ? a : 0u;
         }
         // the key of suffix i at offset h inside its block [.., end): ISA[i + h] + h, or end - 1 - i past the end.
         // Every key's ISA load is in flight together: the index is clamped into the block (and row 0 past the tile's
@@ -224,11 +226,14 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         if (tid == 0) tmax[blockIdx.x] = s_maxsz;
         return;
     }
+    // the rows' suffixes and keys, all loads in flight together: past the tile's rows the loads repeat its last row
+    // (a load under a branch waits for everything before it)
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
-        const uint32_t e = tid + k * 256;
-        gi[k] = e < cnt ? sufl[span0 + e] : 0u;
-        gv[k] = e < cnt ? sec[span0 + e] : 0u;
+        const uint32_t e = tid + k * 256, ec = e < cnt ? e : cnt - 1;
+        const uint32_t a = sufl[span0 + ec], v = sec[span0 + ec];
+        gi[k] = e < cnt ? a : 0u;
+        gv[k] = e < cnt ? v : 0u;
     }
 
     // Direct ranking: each row counts, within its group, the rows of smaller sec (less) and of equal sec before it
