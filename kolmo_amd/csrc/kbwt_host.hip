@@ -308,9 +308,6 @@ inline size_t align16(size_t x) { return (x + 15) & ~(size_t)15; }
 
 // ------------------------------------------------------------------ forward
 
-// bl: one BlockInfo per block with off, n (>= 2), r and soff filled in; packed input dT (N bytes), output dU.
-// I: host array of nsamp sampled indexes. Per-call metadata (blocks, tile tables, code tables) goes to the device
-// in one copy.
 // The rest of the router's features after the doubling rounds: m_r / n from the active counts, and the order-1
 // entropy from bigram histograms (blocks in groups of at most 1024, 256 KiB of histogram each).
 void router_features_tail(Workspace &w, const uint8_t *dT, const BlockInfo *d_bl, const std::vector<BlockInfo> &bl,
@@ -344,6 +341,9 @@ void router_features_tail(Workspace &w, const uint8_t *dT, const BlockInfo *d_bl
     }
 }
 
+// bl: one BlockInfo per block with off, n (>= 2), r and soff filled in; packed input dT (N bytes), output dU.
+// I: host array of nsamp sampled indexes. Per-call metadata (blocks, tile tables, code tables) goes to the device
+// in one copy.
 // feat (host, B x ROUTER_NF, or nullptr): the router's features of every block, computed on the way.
 void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo> &bl, uint32_t N, int32_t *I,
              uint32_t nsamp, hipStream_t s, float *feat = nullptr) {
@@ -383,7 +383,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     grow(w.dbase, (size_t)B * MAXP * 256 * 4);
     if (B > 1) grow(w.blk, NN * 2);
 
-    // metadata layout: blocks | hist tiles | sort tiles | codes; the byte counts come back behind it
+    // metadata layout: blocks | hist tiles | sort tiles | partition tiles | codes; the byte counts come back behind it
     const size_t o_ht = align16(B * sizeof(BlockInfo)), o_st = o_ht + (B + 1) * 4, o_pt = o_st + (B + 1) * 4;
     const size_t o_codes = align16(o_pt + (B + 1) * 4);
     const size_t bytes = o_codes + (size_t)B * 256;

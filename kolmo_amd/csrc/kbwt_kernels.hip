@@ -17,14 +17,17 @@
 //      the sorted keys, so tiles need nothing from each other; ISA[suffix] = head (through pairs partitioned by ISA
 //      window in the first round), the row's BWT byte, and the groups of two or more appended to the next round's
 //      group lists.
-//   5. per doubling round with offset h: k_round_keys gathers (group head, ISA[i+h] + h, or the distance to the
-//      block's end when i+h lies past it) for the active rows, the same radix passes sort them (one flat range: the
-//      heads already keep blocks apart), and k_regroup splits the groups.
+//   5. per doubling round with offset h: groups of at most SEG_GROUP rows sort inside one workgroup each
+//      (kbwt_rounds.hip); for the rows of larger groups k_round_keys gathers (group head, ISA[i+h] + h, or the
+//      distance to the block's end when i+h lies past it), the same radix passes sort them (one flat range: the heads
+//      already keep blocks apart), and k_regroup splits those groups.
 //   6. k_bwt + k_samples: U in libsais' convention per block and I[k] = 1 + rank of suffix k*r.
 // Inverse, per batch of BWTs L with primary indexes P (the end marker's row in the n+1-row matrix of each block):
 //   1. byte histograms -> C[c] = first row of byte c, as a global row.
 //   2. k_lf_pass: LF(row) = C[c] + occ(c, row) for every row, a stable multisplit ranking with lookback.
-//   3. k_walk_count: from every splitter row (row % D == 0, and P) follow LF to the next splitter, recording the
+//   3. k_lf_pairs: per row, the row two LF steps on and the first step's byte (the second step's byte follows from
+//      the landing row and C), so a walk takes two steps per load.
+//   4. k_walk_count: from every splitter row (row % D == 0, and P) follow LF to the next splitter, recording the
 //      sublist length and staging its first bytes; k_rank_jump: synchronous pointer jumping gives each splitter's
 //      text position; k_walk_emit: each sublist's bytes land at their positions.
 #include <type_traits>
