@@ -130,21 +130,24 @@ __device__ __forceinline__ uint32_t next_tile(uint32_t *counter, uint32_t ntiles
     return t;
 }
 
-__device__ __forceinline__ uint64_t lanemask_lt() {
-    uint32_t l = __lane_id();
-    return l ? (~0ull >> (64 - l)) : 0ull;
+// Lanes of mask below this lane (mbcnt: two instructions, no lane mask to build).
+__device__ __forceinline__ uint32_t count_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Peers of this lane: the valid lanes whose 8-bit digit equals d. valid_mask = ballot(valid).
+// Peers of this lane: the valid lanes whose 8-bit digit equals d. valid_mask = ballot(valid). One ballot per digit
+// bit; the lanes that differ from this one in bit b are that ballot XOR the lane's own bit spread over a word, and
+// the peers are the valid lanes that differ in no bit (six vector instructions per bit).
 __device__ __forceinline__ uint64_t match_digit8(uint32_t d, uint64_t valid_mask) {
-    uint64_t peers = valid_mask;
+    uint32_t dlo = 0, dhi = 0;
 #pragma unroll
     for (int b = 0; b < 8; b++) {
-        uint32_t bit = (d >> b) & 1u;
-        uint64_t bm = __ballot(bit);
-        peers &= bit ? bm : ~bm;
+        const uint32_t s = (uint32_t)((int32_t)(d << (31 - b)) >> 31);   // all ones when bit b of d is set
+        const uint64_t m = __ballot(s != 0u);
+        dlo |= (uint32_t)m ^ s;
+        dhi |= (uint32_t)(m >> 32) ^ s;
     }
-    return peers;
+    return valid_mask & ~(((uint64_t)dhi << 32) | dlo);
 }
 
 // Wave inclusive scans with shuffles.

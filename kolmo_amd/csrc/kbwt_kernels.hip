@@ -371,7 +371,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
         uint64_t peers = match_digit8(d, __ballot(valid));
         rk[k] = 0;
         if (valid) {
-            uint32_t before = (uint32_t)__popcll(peers & lanemask_lt());
+            uint32_t before = count_below(peers);
             uint32_t c = s_wcnt[wave][d];
             rk[k] = c + before;
             s_wcnt[wave][d] = c + (uint32_t)__popcll(peers);   // every peer writes the same value
@@ -802,7 +802,7 @@ __global__ __launch_bounds__(BLOCK) void k_lf_pass(const uint8_t *__restrict__ L
         dg[k] = d;
         rk[k] = 0;
         if (valid) {
-            uint32_t before = (uint32_t)__popcll(peers & lanemask_lt());
+            uint32_t before = count_below(peers);
             uint32_t c = s_wcnt[wave][d];
             rk[k] = c + before;
             s_wcnt[wave][d] = c + (uint32_t)__popcll(peers);

@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
             const uint64_t peers = match_digit8(d, __ballot(valid));
             rk[k] = 0;
             if (valid) {
-                const uint32_t before = (uint32_t)__popcll(peers & lanemask_lt());
+                const uint32_t before = count_below(peers);
                 const uint32_t c = s_wcnt[wave][d];
                 rk[k] = c + before;
                 s_wcnt[wave][d] = c + (uint32_t)__popcll(peers);
