@@ -379,7 +379,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     }
     __syncthreads();
 
-    // per digit: wave prefixes, tile total, lookback over earlier tiles of the block
+    // per digit: wave prefixes, tile total (the lookback over earlier tiles of the block follows in the scatter)
     uint32_t tot = 0;
     if (tid < 256) {
 #pragma unroll
@@ -387,16 +387,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
         st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, tr.tb == 0 ? ST_INC : ST_AGG, tot));
     }
     uint32_t tstart = block_excl_sum<BLOCK>(tid < 256 ? tot : 0u, s_scratch, nullptr);
-    if (tid < 256) {
-        uint32_t excl = 0;
-        if (tr.tb > 0) {
-            excl = lookback_win<LB>(status + tid, 256, tile, epoch);
-            st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, ST_INC, excl + tot));
-        }
-        s_tstart[tid] = tstart;
-        const uint32_t dbv = VALS ? db[tid] : woff + (uint32_t)min((uint64_t)tid << wsh, (uint64_t)wn);
-        s_gbase[tid] = dbv + excl - tstart;
-    }
+    if (tid < 256) s_tstart[tid] = tstart;
     __syncthreads();
 
     // local sorted position of each item; keys leave first, then values, through one LDS buffer
@@ -408,12 +399,22 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     }
     // A full tile (every tile but a block's last) takes the same steps without a check per item: straight-line code
     // lets the compiler give every store its own registers, where the checked form reused one set and waited for
-    // each store to finish before the next.
+    // each store to finish before the next. The keys go into LDS before the lookback, so that the LDS writes and the
+    // lookback's round trips overlap; only the global stores need the digit bases it gives.
     auto scatter = [&](auto full_tile) {
         constexpr bool FULL = decltype(full_tile)::value;
 #pragma unroll
         for (int k = 0; k < ITEMS; k++)
             if (FULL || item(k) < cnt) s_buf[lidx[k]] = key[k];
+        if (tid < 256) {
+            uint32_t excl = 0;
+            if (tr.tb > 0) {
+                excl = lookback_win<LB>(status + tid, 256, tile, epoch);
+                st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, ST_INC, excl + tot));
+            }
+            const uint32_t dbv = VALS ? db[tid] : woff + (uint32_t)min((uint64_t)tid << wsh, (uint64_t)wn);
+            s_gbase[tid] = dbv + excl - tstart;
+        }
         __syncthreads();
         uint32_t dg[ITEMS];
 #pragma unroll
