@@ -126,7 +126,7 @@ void launch_hist_keys(const uint64_t *K, uint32_t m, const Shifts &sh, uint32_t 
 uint32_t sort_tile(int cfg);
 void launch_sort_pass(const SortArgs &a, uint32_t ntiles, hipStream_t s);
 void launch_regroup(const RegroupArgs &a, hipStream_t s);
-constexpr int PART_BLOCK = 512, PART_ITEMS = 16, PART_TILE = PART_BLOCK * PART_ITEMS;   // the ISA partition pass
+constexpr int PART_BLOCK = 1024, PART_ITEMS = 16, PART_TILE = PART_BLOCK * PART_ITEMS;   // the ISA partition pass
 void launch_partition_pairs(const uint64_t *in, uint64_t *out, uint32_t N, const Tiling &tl, uint32_t ntiles,
                             uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s);
 void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t *ISA, hipStream_t s);
