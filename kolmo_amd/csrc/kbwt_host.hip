@@ -54,7 +54,7 @@ struct Workspace {
     std::mutex mu;
     uint32_t epoch = 1;
     Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, carry, samples, stage, blk, counts, dbase,
-        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es, ecnt, eoff, pair, tmax;
+        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es, ecnt, eoff, pair, tmax, lrow;
     uint32_t *counters = nullptr;           // 16 tile counters (zeroed once); [15]: the device epoch of graphs
     uint32_t *gcnt = nullptr;               // 4 group-list counters
     uint32_t *h_gcnt = nullptr;
@@ -528,6 +528,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     grow(w.tmax, (NN / SEG_TILE + 2) * 4);
     grow(w.sec, NN * 4);
     grow(w.sufl, NN * 4);
+    grow(w.lrow, NN);
     grow(w.lo, GMAX * 8);
     uint32_t *gcnt = w.gcnt;          // [small groups, large groups] of the list being built
     const uint32_t rank_max = tune.seg_rank;
@@ -581,6 +582,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         sa.sec = P<uint32_t>(w.sec);
         sa.sufl = P<uint32_t>(w.sufl);
         sa.L = L;
+        sa.lrow = P<uint8_t>(w.lrow);
         sa.T = dT;
         sa.blk = blk;
         sa.blocks = d_bl;
@@ -1355,7 +1357,7 @@ void kbwt_release(void) {
                    &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.carry, &w.samples,
                    &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
                    &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
-                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,  &w.sufl,     &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax};
+                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,  &w.sufl,     &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax,   &w.lrow};
     if (w.inv_graph.exec) {
         (void)hipGraphExecDestroy(w.inv_graph.exec);
         w.inv_graph.exec = nullptr;

@@ -99,6 +99,7 @@ struct SegRoundArgs {
     uint32_t *SA, *ISA, *sec;     // sec: each row's key, in list order (row go[g] + k of group g)
     uint32_t *sufl;               // each row's suffix, in list order
     uint8_t *L;
+    uint8_t *lrow;                // first round: each row's BWT byte, in list order (carried from L)
     const uint8_t *T;
     const uint16_t *blk;
     const BlockInfo *blocks;      // every block (blocks[0] when blk == nullptr)

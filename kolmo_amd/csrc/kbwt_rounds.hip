@@ -104,12 +104,19 @@ __global__ __launch_bounds__(256) void k_large_elems(const uint32_t *__restrict_
 // tiles whose splitting groups have at most rank_max rows (empty tiles and tiles where nothing splits included) by
 // direct ranking; its small LDS footprint lets eight workgroups share a CU. MODE 1 splits the other tiles by an LDS
 // radix sort.
-template <int MODE>
+//
+// The BWT byte of a row, T[i - 1] for suffix i, is written to L when the row ends up alone (its SA position is final
+// then). In the first round (CARRY) it needs no read of T: L holds the byte of every row after the first sort, the
+// gather reads it along with SA (lrow, in list order), and every row that moves takes it along (L[r] stays the byte
+// before the suffix at SA row r). Later rounds move rows many times before they end up alone, so there a row reads T
+// once, at the end, instead.
+template <int MODE, bool CARRY>
 __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ gh, const uint32_t *__restrict__ go,
                                                    const uint32_t *__restrict__ tstart, uint32_t G, uint32_t E, uint32_t *__restrict__ SA,
                                                    uint32_t *__restrict__ ISA, uint32_t *__restrict__ sec,
                                                    uint32_t *__restrict__ sufl,
-                                                   uint8_t *__restrict__ L, const uint8_t *__restrict__ T,
+                                                   uint8_t *__restrict__ L, uint8_t *__restrict__ lrow,
+                                                   const uint8_t *__restrict__ T,
                                                    const uint16_t *__restrict__ blk,
                                                    const BlockInfo *__restrict__ blocks, uint32_t n, uint32_t round,
                                                    uint32_t *__restrict__ nh, uint32_t *__restrict__ ns,
@@ -129,6 +136,7 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     __shared__ uint64_t s_buf[RADIX ? 2 * CAP : 1];
     __shared__ uint32_t s_sec[MODE == 2 ? CAP : 1];
     __shared__ uint32_t s_i[MODE ? CAP : 1];
+    __shared__ uint8_t s_lb[RADIX && CARRY ? CAP : 1];   // the radix split: each row's BWT byte by its slot before the sort
     __shared__ uint32_t s_wcnt[RADIX ? W : 1][256];
     __shared__ uint32_t s_start[RADIX ? 256 : 1];
     __shared__ uint32_t s_scratch[16];
@@ -169,10 +177,11 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     __syncthreads();
 
     // gather: row e of the tile belongs to group lg (binary search over the tile's group offsets). MODE 0 reads each
-    // row's suffix from SA and its key at i + h, and stores both in list order (row span0 + e of the round: sufl,
-    // sec), so the split reads them back contiguously. The ITEMS rows of a thread are independent, so their loads
+    // row's suffix from SA, its BWT byte from L and its key at i + h, and stores them in list order (row span0 + e of
+    // the round: sufl, lrow, sec), so the split reads them back contiguously. The ITEMS rows of a thread are independent, so their loads
     // are issued together.
     uint32_t ga[ITEMS], gi[ITEMS], gv[ITEMS];
+    uint8_t gb[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const uint32_t e = tid + k * 256;
@@ -189,6 +198,7 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
             const uint32_t e = tid + k * 256;
             const uint32_t r = s_gh[ga[k]] + (e - s_go[ga[k]]);   // an SA row of the tile's groups below cnt
             const uint32_t a = SA[e < cnt ? r : s_gh[0]];
+            gb[k] = CARRY ? L[e < cnt ? r : s_gh[0]] : (uint8_t)0;
             gi[k] = e < cnt ? a : 0u;
         }
         // the key of suffix i at offset h inside its block [.., end): ISA[i + h] + h, or end - 1 - i past the end.
@@ -213,6 +223,7 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
             const uint32_t e = tid + k * 256;
             if (e < cnt) {
                 sufl[span0 + e] = gi[k];
+                if (CARRY) lrow[span0 + e] = gb[k];
                 sec[span0 + e] = gv[k];
             }
         }
@@ -232,6 +243,7 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     for (int k = 0; k < ITEMS; k++) {
         const uint32_t e = tid + k * 256, ec = e < cnt ? e : cnt - 1;
         const uint32_t a = sufl[span0 + ec], v = sec[span0 + ec];
+        gb[k] = CARRY ? lrow[span0 + ec] : (uint8_t)0;
         gi[k] = e < cnt ? a : 0u;
         gv[k] = e < cnt ? v : 0u;
     }
@@ -239,8 +251,8 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     // Direct ranking: each row counts, within its group, the rows of smaller sec (less) and of equal sec before it
     // (eqb). Its new SA position is head_of_group + less + eqb, its subgroup's head head_of_group + less, and the
     // last row of a subgroup of two or more emits it. Same order as the radix split (rows of equal sec keep their
-    // slot order). Both write nothing for a group that does not split, and the BWT byte of a row only when it ends
-    // up alone (its SA position is final then).
+    // slot order). Both write nothing for a group that does not split; a row that moves writes its suffix at its
+    // new SA position (and in the first round its byte).
     if (MODE == 2) {
         // a group splits when some row's key differs from its first row's; the rows of a group that does not
         // split know their ranks without comparing
@@ -286,9 +298,12 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
                 if (eq != en - st) {             // the group splits (else SA, ISA stay as they are)
                     // only what changes: a row that keeps its SA position or its head (the first subgroup keeps
                     // the group's) writes nothing there
-                    if (pnew != s_gh[lg] + (e - st)) SA[pnew] = i;
+                    if (pnew != s_gh[lg] + (e - st)) {
+                        SA[pnew] = i;
+                        if (CARRY) L[pnew] = gb[k];
+                    }
                     if (less) ISA[i] = head;
-                    if (eq == 1) L[pnew] = i ? T[i - 1] : (uint8_t)0;   // a row's byte is written once it is alone
+                    if (!CARRY && eq == 1) L[pnew] = i ? T[i - 1] : (uint8_t)0;
                 }
                 if (eq >= 2 && eqb == eq - 1) { eh[k] = head; es[k] = eq; nemit++; }
             }
@@ -311,6 +326,7 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
             const uint64_t key = (((uint64_t)ga[k] << 32) | gv[k]) << 12 | e;   // group, sec, row (sort as a unit)
             s_buf[e] = key;
             s_i[e] = gi[k];
+            if (CARRY) s_lb[e] = gb[k];
             ko |= key;
             ka &= key;
         }
@@ -423,9 +439,12 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
             const uint32_t head = s_gh[gk[k]] + (hslot - s_go[gk[k]]);
             hk[k] = head;
             if (s_nsub[gk[k]] > 1) {   // only what changes (see the direct ranking above)
-                if (pk[k] != s_gh[gk[k]] + (ek[k] - s_go[gk[k]])) SA[pk[k]] = ik[k];
+                if (pk[k] != s_gh[gk[k]] + (ek[k] - s_go[gk[k]])) {
+                    SA[pk[k]] = ik[k];
+                    if (CARRY) L[pk[k]] = s_lb[ek[k]];
+                }
                 if (head != s_gh[gk[k]]) ISA[ik[k]] = head;
-                if (flag[k] && lastk[k]) L[pk[k]] = ik[k] ? T[ik[k] - 1] : (uint8_t)0;
+                if (!CARRY && flag[k] && lastk[k]) L[pk[k]] = ik[k] ? T[ik[k] - 1] : (uint8_t)0;
             }
             if (lastk[k] && !flag[k]) nemit++;
         }
@@ -478,12 +497,20 @@ void launch_large_elems(const uint32_t *lh, const uint32_t *lo, uint32_t G, uint
 void launch_seg_round(int mode, const SegRoundArgs &a, hipStream_t s) {
     const uint32_t ntiles = (a.E + SEG_TILE - 1) / SEG_TILE;
     if (!ntiles) return;
-#define KBWT_SEG_ARGS a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.sufl, a.L, a.T, a.blk, a.blocks, a.n, a.round, \
+#define KBWT_SEG_ARGS a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.sufl, a.L, a.lrow, a.T, a.blk, a.blocks, a.n, a.round, \
                       a.nh, a.ns, a.ncnt, a.tmax, a.rank_max
     // every tile goes to exactly one of the two split kernels (modes 1 and 2)
-    if (mode == 0) k_seg_round<0><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
-    else if (mode == 1) k_seg_round<1><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
-    else k_seg_round<2><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
+    const bool carry = a.round == 1;
+    if (mode == 0) {
+        if (carry) k_seg_round<0, true><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
+        else k_seg_round<0, false><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
+    } else if (mode == 1) {
+        if (carry) k_seg_round<1, true><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
+        else k_seg_round<1, false><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
+    } else {
+        if (carry) k_seg_round<2, true><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
+        else k_seg_round<2, false><<<ntiles, 256, 0, s>>>(KBWT_SEG_ARGS);
+    }
 #undef KBWT_SEG_ARGS
 }
 
