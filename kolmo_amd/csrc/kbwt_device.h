@@ -193,8 +193,9 @@ __device__ __forceinline__ void block_scan_sum_max(uint32_t vs, uint32_t vm, uin
     tot_sum = ts; tot_max = tm;
 }
 
-// Block-wide exclusive sum (one value per thread).
-template <int BLOCK>
+// Block-wide exclusive sum (one value per thread). REUSE: end with a barrier so that scratch can be written again
+// right away; a caller that never writes it again in the launch passes false.
+template <int BLOCK, bool REUSE = true>
 __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t *scratch, uint32_t *total) {
     constexpr int W = BLOCK / WAVE;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
@@ -204,7 +205,7 @@ __device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t *scratch
     uint32_t p = 0, t = 0;
 #pragma unroll
     for (int w = 0; w < W; w++) { uint32_t s = scratch[w]; if (w < wave) p += s; t += s; }
-    __syncthreads();
+    if (REUSE) __syncthreads();
     if (total) *total = t;
     return p + is - v;
 }

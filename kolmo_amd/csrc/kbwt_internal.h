@@ -57,7 +57,7 @@ struct Shifts {
 struct SortCfg {
     int block, items, window;
 };
-constexpr int SORT_NCFG = 9;
+constexpr int SORT_NCFG = 11;
 extern const SortCfg SORT_CFGS[SORT_NCFG];
 
 struct SortArgs {

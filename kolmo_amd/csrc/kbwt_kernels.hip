@@ -386,7 +386,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
         for (int w2 = 0; w2 < W; w2++) { uint32_t c = s_wcnt[w2][tid]; s_wcnt[w2][tid] = tot; tot += c; }
         st_store(status + (size_t)tile * 256 + tid, st_pack(epoch, tr.tb == 0 ? ST_INC : ST_AGG, tot));
     }
-    uint32_t tstart = block_excl_sum<BLOCK>(tid < 256 ? tot : 0u, s_scratch, nullptr);
+    uint32_t tstart = block_excl_sum<BLOCK, false>(tid < 256 ? tot : 0u, s_scratch, nullptr);   // s_scratch: once
     if (tid < 256) s_tstart[tid] = tstart;
     __syncthreads();
 
@@ -1124,7 +1124,7 @@ void launch_hist_keys(const uint64_t *K, uint32_t m, const Shifts &sh, uint32_t 
 // The radix pass configurations the tuner chooses from: (workgroup size, items per thread, lookback window).
 const SortCfg SORT_CFGS[SORT_NCFG] = {{512, 16, 4}, {256, 16, 4}, {256, 16, 1}, {512, 8, 4},
                                       {256, 12, 4}, {256, 8, 4}, {512, 16, 1}, {1024, 12, 4},
-                                      {1024, 16, 4}};
+                                      {1024, 16, 4}, {1024, 16, 2}, {1024, 16, 8}};
 uint32_t sort_tile(int cfg) { return (uint32_t)(SORT_CFGS[cfg].block * SORT_CFGS[cfg].items); }
 
 template <int BLOCK, int ITEMS, int LB>
@@ -1149,6 +1149,8 @@ void launch_sort_pass(const SortArgs &a, uint32_t ntiles, hipStream_t s) {
     case 6: sort_pass_cfg<512, 16, 1>(a, ntiles, s); break;
     case 7: sort_pass_cfg<1024, 12, 4>(a, ntiles, s); break;
     case 8: sort_pass_cfg<1024, 16, 4>(a, ntiles, s); break;
+    case 9: sort_pass_cfg<1024, 16, 2>(a, ntiles, s); break;
+    case 10: sort_pass_cfg<1024, 16, 8>(a, ntiles, s); break;
     default: sort_pass_cfg<512, 16, 4>(a, ntiles, s); break;
     }
 }
