@@ -342,8 +342,10 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     __syncthreads();
     const uint64_t vary = ((s_or[0] | s_or[1] | s_or[2] | s_or[3]) ^ (s_and[0] & s_and[1] & s_and[2] & s_and[3])) >> 12;
 
-    // LDS radix sort of the packed keys over the varying 8-bit digits (stable multisplit per pass)
+    // LDS radix sort of the packed keys over the varying 8-bit digits (stable multisplit per pass). Each wave ranks a
+    // contiguous run of ipt * 64 slots, ipt = ceil(cnt / 256): the tile's rows spread evenly over the four waves
     int src = 0;
+    const uint32_t ipt = (cnt + 255) / 256;
     for (uint32_t dsh = 0; dsh < 52; dsh += 8) {
         if (((vary >> dsh) & 0xFFull) == 0) continue;
         const uint32_t shift = 12 + dsh;
@@ -353,17 +355,20 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         uint32_t rk[ITEMS];
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
-            const uint32_t slot = wave * WAVE * ITEMS + k * WAVE + lane;
-            const bool valid = slot < cnt;
-            v[k] = valid ? s_buf[src * CAP + slot] : 0ull;
-            const uint32_t d = (uint32_t)(v[k] >> shift) & 255u;
-            const uint64_t peers = match_digit8(d, __ballot(valid));
+            v[k] = 0;
             rk[k] = 0;
-            if (valid) {
-                const uint32_t before = count_below(peers);
-                const uint32_t c = s_wcnt[wave][d];
-                rk[k] = c + before;
-                s_wcnt[wave][d] = c + (uint32_t)__popcll(peers);
+            if ((uint32_t)k < ipt) {   // uniform over the wave
+                const uint32_t slot = (wave * ipt + k) * WAVE + lane;
+                const bool valid = slot < cnt;
+                v[k] = valid ? s_buf[src * CAP + slot] : 0ull;
+                const uint32_t d = (uint32_t)(v[k] >> shift) & 255u;
+                const uint64_t peers = match_digit8(d, __ballot(valid));
+                if (valid) {
+                    const uint32_t before = count_below(peers);
+                    const uint32_t c = s_wcnt[wave][d];
+                    rk[k] = c + before;
+                    s_wcnt[wave][d] = c + (uint32_t)__popcll(peers);
+                }
             }
         }
         __syncthreads();
@@ -375,8 +380,8 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
-            const uint32_t slot = wave * WAVE * ITEMS + k * WAVE + lane;
-            if (slot < cnt) {
+            const uint32_t slot = (wave * ipt + k) * WAVE + lane;
+            if ((uint32_t)k < ipt && slot < cnt) {
                 const uint32_t d = (uint32_t)(v[k] >> shift) & 255u;
                 s_buf[(src ^ 1) * CAP + s_start[d] + s_wcnt[wave][d] + rk[k]] = v[k];
             }
