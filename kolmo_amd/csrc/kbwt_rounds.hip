@@ -318,36 +318,45 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         return;
     }
 
-    uint64_t ko = 0, ka = ~0ull;
+    // The sort key packs (group, sec, row): the row's slot in the low 12 bits (carried, not sorted), above it the sec
+    // bits that vary over the tile (the bits above the highest varying one are equal in every row, so dropping them
+    // keeps the order), and the group right above those. Only the digits of the sec and group bits are sorted.
+    uint32_t so = 0, sa = ~0u;
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const uint32_t e = tid + k * 256;
         if (e < cnt) {
-            const uint64_t key = (((uint64_t)ga[k] << 32) | gv[k]) << 12 | e;   // group, sec, row (sort as a unit)
-            s_buf[e] = key;
+            so |= gv[k];
+            sa &= gv[k];
+        }
+    }
+    for (int o = 1; o < WAVE; o <<= 1) {
+        so |= __shfl_xor(so, o, WAVE);
+        sa &= __shfl_xor(sa, o, WAVE);
+    }
+    if (lane == 0) { s_or[wave] = so; s_and[wave] = sa; }
+    __syncthreads();
+    const uint32_t svary = (uint32_t)((s_or[0] | s_or[1] | s_or[2] | s_or[3]) ^ (s_and[0] & s_and[1] & s_and[2] & s_and[3]));
+    const uint32_t sbits = svary ? 32u - (uint32_t)__clz(svary) : 0u;               // sec bits kept
+    const uint32_t gbits = ng > 1 ? 32u - (uint32_t)__clz(ng - 1u) : 0u;            // group bits
+    const uint64_t smask = (1ull << sbits) - 1ull;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        const uint32_t e = tid + k * 256;
+        if (e < cnt) {
+            s_buf[e] = (((uint64_t)ga[k] << sbits) | ((uint64_t)gv[k] & smask)) << 12 | e;
             s_i[e] = gi[k];
             if (CARRY) s_lb[e] = gb[k];
-            ko |= key;
-            ka &= key;
         }
     }
     __syncthreads();
+    const uint32_t kbits = sbits + gbits;   // sorted bits above the row index
 
-    // which key bits vary over the tile (bits 12 and up: the row index below is carried, not sorted)
-    for (int o = 1; o < WAVE; o <<= 1) {
-        ko |= __shfl_xor(ko, o, WAVE);
-        ka &= __shfl_xor(ka, o, WAVE);
-    }
-    if (lane == 0) { s_or[wave] = ko; s_and[wave] = ka; }
-    __syncthreads();
-    const uint64_t vary = ((s_or[0] | s_or[1] | s_or[2] | s_or[3]) ^ (s_and[0] & s_and[1] & s_and[2] & s_and[3])) >> 12;
-
-    // LDS radix sort of the packed keys over the varying 8-bit digits (stable multisplit per pass). Each wave ranks a
-    // contiguous run of ipt * 64 slots, ipt = ceil(cnt / 256): the tile's rows spread evenly over the four waves
+    // LDS radix sort of the packed keys over the 8-bit digits of the kept bits (stable multisplit per pass). Each
+    // wave ranks a contiguous run of ipt * 64 slots, ipt = ceil(cnt / 256): the rows spread evenly over the waves
     int src = 0;
     const uint32_t ipt = (cnt + 255) / 256;
-    for (uint32_t dsh = 0; dsh < 52; dsh += 8) {
-        if (((vary >> dsh) & 0xFFull) == 0) continue;
+    for (uint32_t dsh = 0; dsh < kbits; dsh += 8) {
         const uint32_t shift = 12 + dsh;
         for (int i = tid; i < W * 256; i += 256) (&s_wcnt[0][0])[i] = 0;
         __syncthreads();
@@ -409,7 +418,7 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         ek[k] = 0;
         if (slot < cnt) {
             const uint64_t key = buf[slot] >> 12;
-            const uint32_t lg = (uint32_t)(key >> 32);
+            const uint32_t lg = (uint32_t)(key >> sbits);
             gk[k] = lg;
             pk[k] = s_gh[lg] + (slot - s_go[lg]);
             ek[k] = (uint32_t)(buf[slot] & 0xFFFu);   // the row's slot before the sort
