@@ -273,7 +273,8 @@ __global__ __launch_bounds__(256) void k_hist_keys(const uint64_t *__restrict__ 
 // ------------------------------------------------------------------ onesweep radix pass
 
 // Stable pass over (key, value) pairs on the 8-bit digit at `shift`, segmented by block when tl.first is set.
-// FROM_T: keys are built from T with the block's code table (value = the suffix index). dbase[b*dstride + d]: start of
+// FROM_T: keys are built from T with the block's code table (value = the suffix index); this pass and the keys-only
+// one below need not be stable and are not. dbase[b*dstride + d]: start of
 // digit d in block b. status: one lookback word per (tile, digit); counter: tile-id counter.
 // Keys-only passes (VALS == false) move (suffix << 32 | head) pairs alone, partitioned by the suffix's window inside
 // its block b: digit (suffix - off_b) >> window_shift(n_b). The suffixes of a block are a permutation of its own
@@ -362,14 +363,21 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
         }
     }
 
-    // ranking inside each wave in (k, lane) order: stable when that is element order
+    // ranking inside each wave in (k, lane) order: stable when that is element order. Neither the first pass (its
+    // input is in text order) nor the partition of the ISA pairs (their order within a window is free) needs to be
+    // stable, so there an item just takes the next slot of its digit in its wave's counter (one LDS atomic).
+    constexpr bool UNSTABLE = FROM_T || !VALS;
     uint32_t rk[ITEMS];
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         bool valid = item(k) < cnt;
         uint32_t d = digit(key[k]);
-        uint64_t peers = match_digit8(d, __ballot(valid));
         rk[k] = 0;
+        if (UNSTABLE) {
+            if (valid) rk[k] = atomicAdd(&s_wcnt[wave][d], 1u);
+            continue;
+        }
+        uint64_t peers = match_digit8(d, __ballot(valid));
         if (valid) {
             uint32_t before = count_below(peers);
             uint32_t c = s_wcnt[wave][d];
