@@ -52,7 +52,7 @@ KERNEL_BYTES = {
     "scan_groups": 8.0,    # read a group size, write its first-row offset
     "large_elems": 8.0,    # write pos + group head per row of a large group
     "seg_gather": 12.0,    # read the row's suffix 4 and ISA[i+h] 4, write the sort key 4
-    "seg_sort": 18.0,      # read key 4 + suffix 4, write SA 4, ISA 4, row byte 1, read T[i-1] 1
+    "seg_sort": 18.0,      # read key 4 + suffix 4, write SA 4, ISA 4, row byte 1, read it 1 (T[i-1] or carried)
     "isa_part": 16.0,      # read and write one (suffix, head) pair
     "isa_scatter": 12.0,   # read a pair, write ISA
     "emit_compact": 0.0,   # packs the next round's group list (a few bytes per group, not per row)
@@ -74,14 +74,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) and scaled by the same
-    counters on a plain copy, as recorded in profiles/traffic_*.md ('| kernel | ... | bytes per launch |')."""
+def traffic_from_profiles(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) and scaled
+    by the same counters on a plain copy, as recorded in profiles/traffic_<workload>_*.md ('| kernel | ... | bytes per
+    launch |'); None when no profile of that workload holds the kernel."""
     pdir = os.path.join(ROOT, "profiles")
     if not os.path.isdir(pdir):
         return None
     for f in sorted(os.listdir(pdir)):
-        if not (f.startswith("traffic_") and f.endswith(".md")):
+        if not (f.startswith(f"traffic_{workload}_") and f.endswith(".md")):
             continue
         for line in open(os.path.join(pdir, f)):
             m = re.match(r"\|\s*(\S+)\s*\|.*\|\s*([0-9.eE+]+)\s*\|\s*$", line)
@@ -131,14 +132,14 @@ def cpu_baseline(blocks, rs, budget_s=20.0, reps=2):
                       f"inv {inv_s * 1e3:.0f} ms"}
 
 
-def roofline_from(kernels, steps, ms_step, dom, d):
+def roofline_from(kernels, steps, ms_step, dom, d, workload):
     """Roofline of kernel `dom` from its record `d` (summed ms, launches, elements) taken in the timed region; the
     per-kernel table of the whole path comes from the separate all-kernel pass `kernels`."""
     avg_ms = d["ms"] / d["launches"]
     bytes_per_launch = KERNEL_BYTES.get(dom, 0.0) * d["elems"] / d["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom), "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom, workload), "kernel": dom,
                 "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
     path_bytes = sum(KERNEL_BYTES.get(k, 0.0) * v["elems"] for k, v in kernels.items()) / steps
     path = {"bytes_per_step": int(path_bytes), "achieved_GBs": round(path_bytes / (ms_step * 1e-3) / 1e9, 1),
@@ -256,7 +257,7 @@ def run_single(args, K):
     elapsed, kernels, dom, drec = timed_steps(K, args.steps, step)
     ms_step = elapsed / args.steps * 1e3
     value = n * args.steps / elapsed / 2**20
-    roofline, path, dom, achieved = roofline_from(kernels, args.steps, ms_step, dom, drec)
+    roofline, path, dom, achieved = roofline_from(kernels, args.steps, ms_step, dom, drec, args.workload)
     st = K.last_stats()
     mb = model_bytes(n, st)
     path["model_bytes_per_step"] = int(mb)
@@ -334,7 +335,7 @@ def run_batch(args, K):
     torch.cuda.synchronize()
     per_block_s = time.perf_counter() - t0
     per_block = total / per_block_s / 2**20
-    roofline, path, dom, achieved = roofline_from(kernels, args.steps, ms_step, dom, drec)
+    roofline, path, dom, achieved = roofline_from(kernels, args.steps, ms_step, dom, drec, args.workload)
     mb = model_bytes(total, st)
     path["model_bytes_per_step"] = int(mb)
     path["model_frac"] = round(mb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
