@@ -346,7 +346,6 @@ class Tuning(ctypes.Structure):
                 ("ms", ctypes.c_float)]
 
 
-DEFAULT_TUNING = (8, 6, 16)
 
 
 def _tune_sigs(L):
@@ -366,7 +365,7 @@ def autotune(dT, segs, count, start=None, stream=None):
     import torch
     L = _tune_sigs(lib())
     best = Tuning()
-    st = Tuning(*(start or DEFAULT_TUNING), 0.0)
+    st = Tuning(*start, 0.0) if start is not None else Tuning(-1, 0, 0, 0.0)   # not a valid setting: the built-in entry
     s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
     _check(L.kbwt_autotune(dT.data_ptr(), segs, count, s, ctypes.byref(best), ctypes.byref(st)))
     return (best.sort_cfg, best.inv_dshift, best.seg_rank), best.ms, st.ms
@@ -374,6 +373,11 @@ def autotune(dT, segs, count, start=None, stream=None):
 
 def set_tuning(t, size_log2=-1):
     _check(_tune_sigs(lib()).kbwt_set_tuning(size_log2, ctypes.byref(Tuning(*t, 0.0))))
+
+
+def reset_tuning(size_log2=-1):
+    """Back to the library's built-in table (one size class, or all)."""
+    _check(_tune_sigs(lib()).kbwt_set_tuning(size_log2, None))
 
 
 def get_tuning(size_log2):

@@ -278,8 +278,32 @@ struct Tuning {
     uint32_t sort_cfg = 8, inv_dshift = 6, seg_rank = SEG_RANK_MAX;
 };
 constexpr int TUNE_CLASSES = 33;
+// The built-in table, measured on one MI355X over single text + CSV blocks of 1-32 MiB: up to 2 MiB the radix pass
+// takes 8 K-key tiles (more workgroups than CUs) and the inverse walks shorter sublists (splitters every 8 or 16
+// rows: the longest sublist bounds the walk); to 8 MiB splitters every 16 rows; from 16 MiB on 16 K-key tiles and
+// splitters every 64 rows.
+Tuning default_tuning(uint32_t cls) {
+    Tuning t;
+    if (cls <= 20) {
+        t.sort_cfg = 0;
+        t.inv_dshift = 3;
+    } else if (cls == 21) {
+        t.sort_cfg = 0;
+        t.inv_dshift = 4;
+    } else if (cls <= 23) {
+        t.inv_dshift = 4;
+    }
+    return t;
+}
+struct TuneTable {
+    Tuning t[TUNE_CLASSES];
+    TuneTable() {
+        for (int c = 0; c < TUNE_CLASSES; c++) t[c] = default_tuning((uint32_t)c);
+    }
+};
 std::mutex g_tune_mu;
-Tuning g_tune[TUNE_CLASSES];
+TuneTable g_tune_table;
+Tuning (&g_tune)[TUNE_CLASSES] = g_tune_table.t;
 uint32_t size_class(uint32_t N) {
     uint32_t c = 0;
     while (c < 32 && ((uint64_t)1 << c) < N) c++;
@@ -1125,10 +1149,14 @@ static bool tuning_ok(const kbwt_tuning *t) {
 }
 
 int32_t kbwt_set_tuning(int32_t size_log2, const kbwt_tuning *t) {
-    if (!tuning_ok(t) || size_log2 >= TUNE_CLASSES) return KBWT_ERR_ARG;
+    if ((t != NULL && !tuning_ok(t)) || size_log2 >= TUNE_CLASSES) return KBWT_ERR_ARG;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (int c = 0; c < TUNE_CLASSES; c++)
         if (size_log2 < 0 || c == size_log2) {
+            if (t == NULL) {   // back to the built-in table
+                g_tune[c] = default_tuning((uint32_t)c);
+                continue;
+            }
             g_tune[c].sort_cfg = (uint32_t)t->sort_cfg;
             g_tune[c].inv_dshift = (uint32_t)t->inv_dshift;
             g_tune[c].seg_rank = (uint32_t)t->seg_rank;
@@ -1187,7 +1215,7 @@ int32_t kbwt_autotune(const uint8_t *dT, const kbwt_seg *segs, int32_t count, vo
             }
             return best_ms;
         };
-        Tuning cur;
+        Tuning cur = default_tuning(cls);
         if (start && tuning_ok(start)) {
             cur.sort_cfg = (uint32_t)start->sort_cfg;
             cur.inv_dshift = (uint32_t)start->inv_dshift;
@@ -1195,7 +1223,7 @@ int32_t kbwt_autotune(const uint8_t *dT, const kbwt_seg *segs, int32_t count, vo
         }
         float cur_ms = trial(cur);
         const float start_ms = cur_ms;
-        const uint32_t dshifts[] = {4, 5, 6, 7}, ranks[] = {0, 8, 16, 64};
+        const uint32_t dshifts[] = {3, 4, 5, 6, 7}, ranks[] = {0, 8, 16, 64};
         for (int pass = 0; pass < 2; pass++) {
             bool moved = false;
             for (uint32_t c = 0; c < (uint32_t)SORT_NCFG; c++) {

@@ -130,3 +130,25 @@ def test_generators_deterministic():
         b = K.generate(kind, 42, 100000)
         c = K.generate(kind, 43, 100000)
         assert np.array_equal(a, b) and not np.array_equal(a, c), kind
+
+
+def test_tuning_table_set_get_reset():
+    """The per-size-class launch parameters (no device needed): the built-in table, a set, and a reset to it."""
+    try:
+        assert K.get_tuning(20) == (0, 3, 16)      # up to 1 MiB
+        assert K.get_tuning(21) == (0, 4, 16)
+        assert K.get_tuning(23) == (8, 4, 16)
+        assert K.get_tuning(25) == (8, 6, 16)      # 32 MiB and above
+        K.set_tuning((5, 7, 64), 20)
+        assert K.get_tuning(20) == (5, 7, 64) and K.get_tuning(21) == (0, 4, 16)
+        K.reset_tuning(20)
+        assert K.get_tuning(20) == (0, 3, 16)
+        K.set_tuning((1, 9, 0))
+        assert all(K.get_tuning(c) == (1, 9, 0) for c in range(33))
+        with pytest.raises(K.KbwtError):
+            K.set_tuning((11, 6, 16))              # no such radix configuration
+        with pytest.raises(K.KbwtError):
+            K.set_tuning((8, 13, 16))              # splitter spacing out of range
+    finally:
+        K.reset_tuning()
+    assert K.get_tuning(22) == (8, 4, 16) and K.get_tuning(30) == (8, 6, 16)

@@ -27,7 +27,7 @@ def test_gpu_autotune_picks_and_keeps_parity(gpu):
         assert np.array_equal(U, eU) and np.array_equal(I, eI)
         assert np.array_equal(K.unbwt_aux(U, r, I), T)
     finally:
-        K.set_tuning(K.DEFAULT_TUNING)
+        K.reset_tuning()
 
 
 @pytest.mark.parametrize("t", [(1, 4, 0), (2, 5, 8), (3, 7, 64), (4, 6, 1024), (5, 3, 16), (6, 8, 2)])
@@ -42,4 +42,4 @@ def test_gpu_every_setting_is_exact(gpu, t):
             assert np.array_equal(U, eU) and np.array_equal(I, eI), (t, kind)
             assert np.array_equal(K.unbwt_aux(U, r, I), T), (t, kind)
     finally:
-        K.set_tuning(K.DEFAULT_TUNING)
+        K.reset_tuning()
