@@ -431,7 +431,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     HIPCHK(hipEventRecord(w.ev0, s));
     // histogram accumulators start at zero (every scan clears them; this covers a call that stopped half way)
     launch_fill_u8(P<uint8_t>(w.counts), 0, (size_t)B * MAXP * 256 * 4, s);
-    if (B > 1) TIMED("fill_blk", N, launch_fill_blk(P<uint16_t>(w.blk), d_bl, B, N, s));
+    if (B > 1) TIMED("fill_blk", N, launch_fill_blk(P<uint16_t>(w.blk), thist, nht, s));
     // 1. byte histograms -> dense symbol codes per block
     TIMED("byte_hist", N, launch_byte_hist(dT, thist, nht, P<uint32_t>(w.counts), s));
     TIMED("scan_digits", 256 * B,
@@ -811,7 +811,7 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
               {"fill_blk", "byte_hist", "scan_digits", "lf_pass", "lf_pairs", "walk_count", "rank_jump", "walk_emit"}, [&] {
         launch_epoch_bump(depoch, s);
         launch_fill_u8(P<uint8_t>(w.counts), 0, (size_t)B * MAXP * 256 * 4, s);
-        if (B > 1) TIMED("fill_blk", N, launch_fill_blk(P<uint16_t>(w.blk), d_bl, B, N, s));
+        if (B > 1) TIMED("fill_blk", N, launch_fill_blk(P<uint16_t>(w.blk), thist, nht, s));
         TIMED("byte_hist", N, launch_byte_hist(dL, thist, nht, P<uint32_t>(w.counts), s));
         TIMED("scan_digits", 256 * B,
               launch_scan_digits(P<uint32_t>(w.counts), B, 1, d_bl, 2, nullptr, P<uint32_t>(w.cbase), s));

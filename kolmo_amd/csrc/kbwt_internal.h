@@ -138,7 +138,7 @@ void launch_bwt(const uint8_t *T, const uint8_t *L, const uint32_t *ISA, const u
                 uint8_t *U, uint32_t N, hipStream_t s);
 void launch_samples(const uint32_t *ISA, const BlockInfo *blocks, uint32_t B, uint32_t count, int32_t *I,
                     hipStream_t s);
-void launch_fill_blk(uint16_t *blk, const BlockInfo *blocks, uint32_t B, uint32_t N, hipStream_t s);
+void launch_fill_blk(uint16_t *blk, Tiling tl, uint32_t ntiles, hipStream_t s);   // tiles of HIST_TILE indexes
 void launch_lf_pass(const uint8_t *L, Tiling tl, uint32_t ntiles, const uint32_t *cbase, uint32_t *lf,
                     uint64_t *status, uint32_t *counter, uint32_t epoch, const uint32_t *depoch, hipStream_t s);
 // Lookback epochs: the host numbers launches in [1, EPOCH_HALF); a captured graph reads a device counter in

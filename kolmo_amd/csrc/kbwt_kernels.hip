@@ -768,11 +768,13 @@ __global__ __launch_bounds__(256) void k_samples(const uint32_t *__restrict__ IS
     I[g] = (int32_t)(ISA[bi.off + k * bi.r] - bi.off + 1u);
 }
 
-// blk[o] = the block of packed index o (grid.y = block).
-__global__ __launch_bounds__(256) void k_fill_blk(uint16_t *__restrict__ blk, const BlockInfo *__restrict__ blocks) {
-    const BlockInfo bi = blocks[blockIdx.y];
-    for (size_t o = (size_t)blockIdx.x * 256 + threadIdx.x; o < bi.n; o += (size_t)gridDim.x * 256)
-        blk[bi.off + o] = (uint16_t)blockIdx.y;
+// blk[o] = the block of packed index o: one workgroup per HIST_TILE indexes of a block (the text histograms' tiles).
+__global__ __launch_bounds__(256) void k_fill_blk(uint16_t *__restrict__ blk, Tiling tl) {
+    const uint32_t b = tile_block(tl.first, tl.B, blockIdx.x);
+    const BlockInfo bi = tl.blocks[b];
+    const size_t beg = (size_t)bi.off + (size_t)(blockIdx.x - tl.first[b]) * HIST_TILE;
+    const size_t end = beg + HIST_TILE < (size_t)bi.off + bi.n ? beg + HIST_TILE : (size_t)bi.off + bi.n;
+    for (size_t o = beg + threadIdx.x; o < end; o += 256) blk[o] = (uint16_t)b;
 }
 
 // ------------------------------------------------------------------ inverse BWT
@@ -1198,9 +1200,8 @@ void launch_samples(const uint32_t *ISA, const BlockInfo *blocks, uint32_t B, ui
                     hipStream_t s) {
     if (count) k_samples<<<(count + 255) / 256, 256, 0, s>>>(ISA, blocks, B, count, I);
 }
-void launch_fill_blk(uint16_t *blk, const BlockInfo *blocks, uint32_t B, uint32_t N, hipStream_t s) {
-    (void)N;
-    if (B) k_fill_blk<<<dim3(64, B), 256, 0, s>>>(blk, blocks);
+void launch_fill_blk(uint16_t *blk, Tiling tl, uint32_t ntiles, hipStream_t s) {
+    if (ntiles) k_fill_blk<<<ntiles, 256, 0, s>>>(blk, tl);
 }
 void launch_lf_pass(const uint8_t *L, Tiling tl, uint32_t ntiles, const uint32_t *cbase, uint32_t *lf,
                     uint64_t *status, uint32_t *counter, uint32_t epoch, const uint32_t *depoch, hipStream_t s) {
