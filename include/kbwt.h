@@ -10,7 +10,9 @@
  * (host or device); -3 a HIP runtime error (no usable device, a failed launch). There is no CPU fallback inside this
  * library: a device error is reported, on stderr and through kbwt_last_error(), never hidden.
  *
- * Every function is reentrant; calls from several host threads are serialised per device. */
+ * Every function is reentrant; calls from several host threads are serialised per device, except that concurrent
+ * kolmo_sais_* calls (kolmo's block workers) are gathered into batches: one caller runs every block queued at that
+ * moment as one kbwt batch, and each block still gets exactly its single-block result. */
 #ifndef KBWT_H
 #define KBWT_H
 #include <stddef.h>
@@ -127,7 +129,8 @@ KBWT_API int32_t kbwt_autotune(const uint8_t *dT, const kbwt_seg *segs, int32_t 
 KBWT_API int32_t kbwt_set_tuning(int32_t size_log2, const kbwt_tuning *t);
 KBWT_API int32_t kbwt_get_tuning(int32_t size_log2, kbwt_tuning *t);
 
-/* ---- telemetry of the calling thread's last call ---- */
+/* ---- telemetry of the calling thread's last call (a kolmo_sais_* block that ran in another caller's batch leaves
+   it with that caller's thread) ---- */
 #define KBWT_MAX_ROUNDS 40
 typedef struct {
     int32_t n;                         /* block length (a batch: the blocks' total length) */

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -1026,6 +1027,104 @@ template <class F> static int32_t guarded(F f) {
     }
 }
 
+// ---- concurrent single-block calls run as batches
+//
+// kolmo's block workers call kolmo_sais_bwt_aux and kolmo_sais_unbwt_aux concurrently (src/core/compress.c:231-256,
+// src/core/decode.c:404-410). A call queues its block; a caller that finds no batch running takes every queued block
+// (up to the batch limits), runs the forward blocks as one batch and the inverse blocks as another, and wakes the
+// others. Each block's result is the single-block result, so batching changes nothing but the launch count.
+struct Pending {
+    bool fwd;
+    const uint8_t *T;
+    uint8_t *U;
+    int32_t n, r;        // r: the sampling interval (forward) or r of the inverse
+    int32_t *I;          // forward: the sampled indexes out
+    const int32_t *Iin;  // inverse: the sampled indexes in
+    int32_t rc = 0;
+    bool done = false;
+    std::string err;
+};
+static std::mutex g_comb_mu;
+static std::condition_variable g_comb_cv;
+static std::vector<Pending *> g_comb_q;
+static bool g_comb_busy = false;
+
+// One batch of one kind from host buffers: blocks packed back to back in the workspace's staging buffers.
+static void run_pending(std::vector<Pending *> &b, bool fwd) {
+    if (b.empty()) return;
+    const int32_t count = (int32_t)b.size();
+    std::vector<kbwt_seg> segs(count);
+    int64_t at = 0;
+    size_t ns = 0;
+    for (int32_t k = 0; k < count; k++) {
+        segs[k].offset = at;
+        segs[k].n = b[k]->n;
+        segs[k].r = b[k]->r;
+        at += b[k]->n;
+        ns += nsamples(b[k]->n, b[k]->r);
+    }
+    std::vector<int32_t> I(ns);
+    if (!fwd)
+        for (int32_t k = 0, o = 0; k < count; o += (int32_t)nsamples(b[k]->n, b[k]->r), k++)
+            memcpy(I.data() + o, b[k]->Iin, nsamples(b[k]->n, b[k]->r) * 4);
+    const int32_t rc = guarded([&] {
+        Workspace &w = workspace();
+        std::lock_guard<std::mutex> lk(w.mu);
+        grow(w.T, (size_t)at + 64);
+        if (fwd) grow(w.U, (size_t)at + 64);
+        for (int32_t k = 0; k < count; k++)
+            HIPCHK(hipMemcpyAsync(P<uint8_t>(w.T) + segs[k].offset, b[k]->T, (size_t)b[k]->n, hipMemcpyHostToDevice,
+                                  w.stream));
+        uint8_t *out = fwd ? P<uint8_t>(w.U) : P<uint8_t>(w.T);
+        if (fwd) forward_batch_dev(w, P<uint8_t>(w.T), out, segs.data(), count, I.data(), w.stream);
+        else inverse_batch_dev(w, P<uint8_t>(w.T), out, segs.data(), count, I.data(), w.stream);
+        for (int32_t k = 0; k < count; k++)
+            HIPCHK(hipMemcpyAsync(b[k]->U, out + segs[k].offset, (size_t)b[k]->n, hipMemcpyDeviceToHost, w.stream));
+        HIPCHK(hipStreamSynchronize(w.stream));
+    });
+    const std::string err = g_last_error;
+    for (int32_t k = 0, o = 0; k < count; o += (int32_t)nsamples(b[k]->n, b[k]->r), k++) {
+        b[k]->rc = rc;
+        b[k]->err = err;
+        if (fwd && rc == 0) memcpy(b[k]->I, I.data() + o, nsamples(b[k]->n, b[k]->r) * 4);
+    }
+}
+
+static int32_t combine(Pending &p) {
+    std::unique_lock<std::mutex> lk(g_comb_mu);
+    g_comb_q.push_back(&p);
+    while (!p.done) {
+        if (g_comb_busy) {
+            g_comb_cv.wait(lk);
+            continue;
+        }
+        // take the queue's head up to the batch limits (65535 blocks, packed length below 2^31)
+        g_comb_busy = true;
+        std::vector<Pending *> fwd, inv, taken;
+        int64_t tf = 0, ti = 0;
+        size_t k = 0;
+        for (; k < g_comb_q.size(); k++) {
+            Pending *q = g_comb_q[k];
+            int64_t &t = q->fwd ? tf : ti;
+            std::vector<Pending *> &v = q->fwd ? fwd : inv;
+            if (v.size() == 65535 || t + q->n > INT32_MAX - 64) break;
+            t += q->n;
+            v.push_back(q);
+            taken.push_back(q);
+        }
+        g_comb_q.erase(g_comb_q.begin(), g_comb_q.begin() + (long)k);
+        lk.unlock();
+        run_pending(fwd, true);
+        run_pending(inv, false);
+        lk.lock();
+        for (Pending *q : taken) q->done = true;
+        g_comb_busy = false;
+        g_comb_cv.notify_all();
+    }
+    g_last_error = p.rc ? p.err : std::string();
+    return p.rc;
+}
+
 extern "C" {
 
 int32_t kolmo_sais_bwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t r, int32_t *I,
@@ -1037,21 +1136,15 @@ int32_t kolmo_sais_bwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, 
         I[0] = n;
         return 0;
     }
-    return guarded([&] {
-        Workspace &w = workspace();
-        std::lock_guard<std::mutex> lk(w.mu);
-        grow(w.T, (size_t)n + 64);
-        grow(w.U, (size_t)n + 64);
-        HIPCHK(hipMemcpyAsync(w.T.p, T, (size_t)n, hipMemcpyHostToDevice, w.stream));
-        std::vector<BlockInfo> bl(1);
-        bl[0].off = 0;
-        bl[0].n = (uint32_t)n;
-        bl[0].r = (uint32_t)r;
-        bl[0].soff = 0;
-        forward(w, P<uint8_t>(w.T), P<uint8_t>(w.U), bl, (uint32_t)n, I, nsamples(n, r), w.stream);
-        HIPCHK(hipMemcpyAsync(U, w.U.p, (size_t)n, hipMemcpyDeviceToHost, w.stream));
-        HIPCHK(hipStreamSynchronize(w.stream));
-    });
+    Pending p;
+    p.fwd = true;
+    p.T = T;
+    p.U = U;
+    p.n = n;
+    p.r = r;
+    p.I = I;
+    p.Iin = nullptr;
+    return combine(p);
 }
 
 int32_t kolmo_sais_unbwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t r, const int32_t *I,
@@ -1066,19 +1159,15 @@ int32_t kolmo_sais_unbwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n
     }
     for (int64_t t = 0; t <= (n - 1) / r; ++t)
         if (I[t] <= 0 || I[t] > n) return -1;
-    return guarded([&] {
-        Workspace &w = workspace();
-        std::lock_guard<std::mutex> lk(w.mu);
-        grow(w.T, (size_t)n + 64);
-        HIPCHK(hipMemcpyAsync(w.T.p, T, (size_t)n, hipMemcpyHostToDevice, w.stream));
-        std::vector<BlockInfo> bl(1);
-        bl[0].off = 0;
-        bl[0].n = (uint32_t)n;
-        bl[0].P = (uint32_t)I[0];
-        inverse(w, P<uint8_t>(w.T), P<uint8_t>(w.T), bl, (uint32_t)n, w.stream);
-        HIPCHK(hipMemcpyAsync(U, w.T.p, (size_t)n, hipMemcpyDeviceToHost, w.stream));
-        HIPCHK(hipStreamSynchronize(w.stream));
-    });
+    Pending p;
+    p.fwd = false;
+    p.T = T;
+    p.U = U;
+    p.n = n;
+    p.r = r;
+    p.I = nullptr;
+    p.Iin = I;
+    return combine(p);
 }
 
 int32_t kolmo_sais_unbwt(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t i, int32_t threads) {
