@@ -1050,7 +1050,19 @@ static std::vector<Pending *> g_comb_q;
 static bool g_comb_busy = false;
 
 // One batch of one kind from host buffers: blocks packed back to back in the workspace's staging buffers.
+static void run_pending_batch(std::vector<Pending *> &b, bool fwd);
+// Never throws: a caller that runs a batch must always wake the others.
 static void run_pending(std::vector<Pending *> &b, bool fwd) {
+    try {
+        run_pending_batch(b, fwd);
+    } catch (...) {
+        for (Pending *q : b) {
+            q->rc = KBWT_ERR_NOMEM;
+            q->err = "host allocation failed";
+        }
+    }
+}
+static void run_pending_batch(std::vector<Pending *> &b, bool fwd) {
     if (b.empty()) return;
     const int32_t count = (int32_t)b.size();
     std::vector<kbwt_seg> segs(count);
@@ -1092,15 +1104,30 @@ static void run_pending(std::vector<Pending *> &b, bool fwd) {
 
 static int32_t combine(Pending &p) {
     std::unique_lock<std::mutex> lk(g_comb_mu);
-    g_comb_q.push_back(&p);
+    try {
+        g_comb_q.push_back(&p);
+    } catch (...) {
+        g_last_error = "host allocation failed";
+        return KBWT_ERR_NOMEM;
+    }
     while (!p.done) {
         if (g_comb_busy) {
             g_comb_cv.wait(lk);
             continue;
         }
-        // take the queue's head up to the batch limits (65535 blocks, packed length below 2^31)
-        g_comb_busy = true;
+        // take the queue's head up to the batch limits (65535 blocks, packed length below 2^31); nothing below can
+        // fail once the lists are reserved, so the runner always wakes the others
         std::vector<Pending *> fwd, inv, taken;
+        try {
+            fwd.reserve(g_comb_q.size());
+            inv.reserve(g_comb_q.size());
+            taken.reserve(g_comb_q.size());
+        } catch (...) {
+            g_comb_q.erase(std::remove(g_comb_q.begin(), g_comb_q.end(), &p), g_comb_q.end());
+            g_last_error = "host allocation failed";
+            return KBWT_ERR_NOMEM;
+        }
+        g_comb_busy = true;
         int64_t tf = 0, ti = 0;
         size_t k = 0;
         for (; k < g_comb_q.size(); k++) {

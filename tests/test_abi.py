@@ -152,3 +152,34 @@ def test_tuning_table_set_get_reset():
     finally:
         K.reset_tuning()
     assert K.get_tuning(22) == (8, 4, 16) and K.get_tuning(30) == (8, 6, 16)
+
+
+def test_concurrent_calls_without_device_fail_cleanly():
+    """Concurrent kolmo_sais_* calls are gathered into batches by whichever caller finds none running. Without a
+    device every batch fails, and every call (forward and inverse alike) must come back with -3, none left waiting."""
+    if K.device_ok():
+        pytest.skip("a device is present")
+    import threading
+    T = np.frombuffer(b"abracadabra" * 100, np.uint8).copy()
+    codes, lock = [], threading.Lock()
+
+    def call(k):
+        for _ in range(5):
+            try:
+                if k % 2:
+                    K.bwt_aux(T, 64)
+                else:
+                    K.unbwt_aux(T, 64, np.array([1] + [2] * 17, np.int32))
+                rc = 0
+            except K.KbwtError as e:
+                rc = e.code
+            with lock:
+                codes.append(rc)
+
+    ts = [threading.Thread(target=call, args=(k,)) for k in range(16)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=60)
+    assert not any(t.is_alive() for t in ts), "a caller is still waiting"
+    assert len(codes) == 80 and set(codes) == {K.KBWT_ERR_DEVICE}
