@@ -19,7 +19,9 @@ measured with HIP events on its own stream inside the timed region (roofline), a
 (oracle/_ref, compiled from kolmo's sources) timed on this host's cores on a bounded sample of the same workload.
 The default cfg2 line also carries, under config: the batched form on cfg4 against one call per block ("batched"),
 the shipped router on the held-out router blocks ("router"), and the cfg2 block through the host-pointer entry
-points, PCIe included ("pcie_inclusive"); --no-extras leaves them out.
+points, PCIe included ("pcie_inclusive"), and the same block as 1 MiB blocks through those entry points from 16
+concurrent callers as kolmo's block workers make them, against one caller ("concurrent_calls"); --no-extras leaves
+them out.
 """
 import argparse
 import json
@@ -369,6 +371,61 @@ def pcie_inclusive(K, reps=3):
             "what": "kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux on host buffers, best of 3"}
 
 
+def concurrent_calls(K, workers=16, reps=3):
+    """kolmo's block workers on the host-pointer entry points: the cfg2 block cut into 1 MiB blocks, each worker
+    taking blocks in turn and calling kolmo_sais_bwt_aux then kolmo_sais_unbwt_aux (PCIe included), against the same
+    calls made one after another from one thread. Concurrent calls run as batches inside the library."""
+    import threading
+    import numpy as np
+    gen, seed, n, _ = WORKLOADS["cfg2"]
+    T = K.generate(gen, seed, n)
+    blocks = [T[o:o + (1 << 20)].copy() for o in range(0, n, 1 << 20)]
+    rs = [1 << K.codec18_rlog(b.size) for b in blocks]
+
+    def one(k):
+        U, I = K.bwt_aux(blocks[k], rs[k])
+        back = K.unbwt_aux(U, rs[k], I)
+        if not np.array_equal(back, blocks[k]):
+            raise SystemExit("round trip failed in the concurrent-call measurement")
+
+    def serial():
+        for k in range(len(blocks)):
+            one(k)
+
+    def parallel():
+        ks = list(range(len(blocks)))
+        lock = threading.Lock()
+
+        def worker():
+            while True:
+                with lock:
+                    if not ks:
+                        return
+                    k = ks.pop(0)
+                one(k)
+
+        ts = [threading.Thread(target=worker) for _ in range(workers)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+
+    res = {}
+    for name, f in (("one_thread", serial), ("workers", parallel)):
+        best = None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        res[name] = round(n / best / 2**20, 2)
+    log(f"bench: 1 MiB blocks through kolmo_sais_*: {res['one_thread']} MiB/s from one thread, "
+        f"{res['workers']} MiB/s from {workers} workers")
+    return {"blocks": len(blocks), "block_bytes": 1 << 20, "workers": workers, "unit": "MiB/s",
+            "one_thread": res["one_thread"], "concurrent_workers": res["workers"],
+            "what": "kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux per block on host buffers (PCIe included), best of 3"}
+
+
 def router_summary():
     """The shipped router on the held-out fifth of the router workload (8192 blocks of 256 KiB-4 MiB, seed 5): its
     bytes against exhaustive best-per-block and its coder throughput, from the committed labels and GPU features."""
@@ -421,6 +478,7 @@ def main():
     value, ms_step, config, roofline, path, cpu = run(args, K)
     if args.workload == "cfg2" and not args.no_extras:
         config["pcie_inclusive"] = pcie_inclusive(K)
+        config["concurrent_calls"] = concurrent_calls(K)
         bargs = argparse.Namespace(**vars(args))
         bargs.workload, bargs.steps, bargs.warmup, bargs.no_cpu = "cfg4", 2, 1, True
         bv, bms, bcfg, _, _, _ = run_batch(bargs, K)
