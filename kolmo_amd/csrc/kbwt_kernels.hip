@@ -424,23 +424,29 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
             s_gbase[tid] = dbv + excl - tstart;
         }
         __syncthreads();
+        // the sorted keys come out of LDS into registers first, so that the values can go into the same LDS while
+        // the keys' global stores are issued
         uint32_t dg[ITEMS];
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
             const uint32_t j = tid + k * BLOCK;
-            dg[k] = 0;
-            if (FULL || j < cnt) {
-                const uint64_t kk = s_buf[j];
-                dg[k] = digit(kk);
-                kout[s_gbase[dg[k]] + j] = kk;
-            }
+            key[k] = FULL || j < cnt ? s_buf[j] : 0ull;
+            dg[k] = digit(key[k]);
+        }
+        if (VALS) {
+            __syncthreads();
+            uint32_t *s_v = reinterpret_cast<uint32_t *>(s_buf);
+#pragma unroll
+            for (int k = 0; k < ITEMS; k++)
+                if (FULL || item(k) < cnt) s_v[lidx[k]] = val[k];
+        }
+#pragma unroll
+        for (int k = 0; k < ITEMS; k++) {
+            const uint32_t j = tid + k * BLOCK;
+            if (FULL || j < cnt) kout[s_gbase[dg[k]] + j] = key[k];
         }
         if (!VALS) return;
-        __syncthreads();
         uint32_t *s_v = reinterpret_cast<uint32_t *>(s_buf);
-#pragma unroll
-        for (int k = 0; k < ITEMS; k++)
-            if (FULL || item(k) < cnt) s_v[lidx[k]] = val[k];
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < ITEMS; k++) {
