@@ -1,5 +1,6 @@
 // Host side of libkbwt_hip: the C-ABI entry points of include/kbwt.h, the per-device workspace, and the steps of the
-// forward suffix sort (round loop) and of the inverse, for a batch of blocks (one block is the batch of one).
+// forward suffix sort (round loop) and of the inverse, for a batch of blocks (one block is the batch of one);
+// concurrent kolmo_sais_* calls are gathered into batches; the per-size-class tuning table and the autotuner.
 // Validation and return codes follow src/algo/sais/sais.c:12027-12082 of kolmo 0.0.2.
 #include "kbwt_internal.h"
 #include "kbwt_router.h"
@@ -55,7 +56,8 @@ struct Workspace {
     std::mutex mu;
     uint32_t epoch = 1;
     Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, carry, samples, stage, blk, counts, dbase,
-        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es, ecnt, eoff, pair, tmax, lrow;
+        tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es,
+        ecnt, eoff, pair, tmax, lrow;
     uint32_t *counters = nullptr;           // 16 tile counters (zeroed once); [15]: the device epoch of graphs
     uint32_t *gcnt = nullptr;               // 4 group-list counters
     uint32_t *h_gcnt = nullptr;
@@ -1501,7 +1503,8 @@ void kbwt_release(void) {
                    &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.carry, &w.samples,
                    &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
                    &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
-                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,  &w.sufl,     &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax,   &w.lrow};
+                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,
+                   &w.sufl,    &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax,   &w.lrow};
     if (w.inv_graph.exec) {
         (void)hipGraphExecDestroy(w.inv_graph.exec);
         w.inv_graph.exec = nullptr;

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     __shared__ uint64_t s_buf[RADIX ? 2 * CAP : 1];
     __shared__ uint32_t s_sec[MODE == 2 ? CAP : 1];
     __shared__ uint32_t s_i[MODE ? CAP : 1];
-    __shared__ uint8_t s_lb[RADIX && CARRY ? CAP : 1];   // the radix split: each row's BWT byte by its slot before the sort
+    __shared__ uint8_t s_lb[RADIX && CARRY ? CAP : 1];   // radix split: each row's BWT byte by its slot before the sort
     __shared__ uint32_t s_wcnt[RADIX ? W : 1][256];
     __shared__ uint32_t s_start[RADIX ? 256 : 1];
     __shared__ uint32_t s_scratch[16];
@@ -336,7 +336,8 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
     }
     if (lane == 0) { s_or[wave] = so; s_and[wave] = sa; }
     __syncthreads();
-    const uint32_t svary = (uint32_t)((s_or[0] | s_or[1] | s_or[2] | s_or[3]) ^ (s_and[0] & s_and[1] & s_and[2] & s_and[3]));
+    const uint32_t svary =
+        (uint32_t)((s_or[0] | s_or[1] | s_or[2] | s_or[3]) ^ (s_and[0] & s_and[1] & s_and[2] & s_and[3]));
     const uint32_t sbits = svary ? 32u - (uint32_t)__clz(svary) : 0u;               // sec bits kept
     const uint32_t gbits = ng > 1 ? 32u - (uint32_t)__clz(ng - 1u) : 0u;            // group bits
     const uint64_t smask = (1ull << sbits) - 1ull;
@@ -511,8 +512,8 @@ void launch_large_elems(const uint32_t *lh, const uint32_t *lo, uint32_t G, uint
 void launch_seg_round(int mode, const SegRoundArgs &a, hipStream_t s) {
     const uint32_t ntiles = (a.E + SEG_TILE - 1) / SEG_TILE;
     if (!ntiles) return;
-#define KBWT_SEG_ARGS a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.sufl, a.L, a.lrow, a.T, a.blk, a.blocks, a.n, a.round, \
-                      a.nh, a.ns, a.ncnt, a.tmax, a.rank_max
+#define KBWT_SEG_ARGS a.gh, a.go, a.tstart, a.G, a.E, a.SA, a.ISA, a.sec, a.sufl, a.L, a.lrow, a.T, a.blk, a.blocks, \
+                      a.n, a.round, a.nh, a.ns, a.ncnt, a.tmax, a.rank_max
     // every tile goes to exactly one of the two split kernels (modes 1 and 2)
     const bool carry = a.round == 1;
     if (mode == 0) {
