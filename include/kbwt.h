@@ -170,7 +170,7 @@ KBWT_API const char *kbwt_last_error(void);
 /* 1 when a gfx950 device is usable, else 0. */
 KBWT_API int32_t kbwt_device_ok(void);
 
-/* Releases the cached device workspace of the current device. */
+/* Releases the cached device workspace of the current device, and its pinned host staging chunks. */
 KBWT_API void kbwt_release(void);
 
 #ifdef __cplusplus

@@ -17,6 +17,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace kbwt {
@@ -69,6 +70,8 @@ struct Workspace {
         hipGraphExec_t exec = nullptr;
     } inv_graph;
     // pinned host staging
+    uint8_t *pin[2] = {nullptr, nullptr};   // the host-pointer entry points' double buffer (PIN_CHUNK bytes each)
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
     void *h_meta = nullptr;
     size_t h_meta_cap = 0;
     unsigned long long *h_orand = nullptr, *h_orand_init = nullptr;
@@ -157,14 +160,196 @@ Workspace &workspace() {
         w.h_orand_init[0] = 0ull;
         w.h_orand_init[1] = ~0ull;
         {   // the device epoch starts in the upper half of the epoch space
-        const uint32_t e0 = EPOCH_HALF;
-        HIPCHK(hipMemcpy(w.counters + 15, &e0, 4, hipMemcpyHostToDevice));
-    }
-    HIPCHK(hipEventCreate(&w.ev0));
+            const uint32_t e0 = EPOCH_HALF;
+            HIPCHK(hipMemcpy(w.counters + 15, &e0, 4, hipMemcpyHostToDevice));
+        }
+        HIPCHK(hipEventCreate(&w.ev0));
         HIPCHK(hipEventCreate(&w.ev1));
         w.ready = true;
     }
     return w;
+}
+
+// ---- host buffers to and from the device through pinned memory
+//
+// The runtime moves pageable host memory through its own staging with one CPU thread. The host-pointer entry points
+// instead move their host spans (the blocks of a call or of a combined batch, back to back on the device) through two
+// pinned chunks: a small pool of CPU threads copies one chunk while the DMA engine moves the other.
+constexpr size_t PIN_CHUNK = (size_t)8 << 20;
+constexpr size_t PIN_MIN = (size_t)4 << 20;   // below this many bytes in all the runtime's own path is as fast
+
+struct Span {   // host bytes
+    uint8_t *p;
+    size_t n;
+};
+struct CopySeg {
+    uint8_t *dst;
+    const uint8_t *src;
+    size_t n;
+};
+
+// memcpy of a list of segments split over a few threads (the caller takes part); one list at a time.
+class CopyPool {
+public:
+    void copy(const std::vector<CopySeg> &segs) {
+        constexpr size_t PART = (size_t)512 << 10;
+        std::lock_guard<std::mutex> one(call_mu_);
+        parts_.clear();
+        for (const CopySeg &g : segs)
+            for (size_t o = 0; o < g.n; o += PART) parts_.push_back({g.dst + o, g.src + o, std::min(PART, g.n - o)});
+        if (parts_.size() < 2) {
+            for (const CopySeg &g : parts_) memcpy(g.dst, g.src, g.n);
+            return;
+        }
+        start();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            next_ = 0;
+            done_ = 0;
+            gen_++;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return done_ == parts_.size(); });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+
+private:
+    void start() {   // under call_mu_
+        if (!th_.empty()) return;
+        const unsigned hw = std::thread::hardware_concurrency();
+        const unsigned k = hw > 2 ? std::min(7u, hw - 1) : 1u;
+        for (unsigned i = 0; i < k; i++) th_.emplace_back([this] { loop(); });
+    }
+    void loop() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+        }
+    }
+    void work() {   // take parts until none is left
+        for (;;) {
+            size_t k;
+            {
+                std::lock_guard<std::mutex> lk(mu_);
+                if (next_ >= parts_.size()) return;
+                k = next_++;
+            }
+            memcpy(parts_[k].dst, parts_[k].src, parts_[k].n);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (++done_ == parts_.size()) done_cv_.notify_all();
+        }
+    }
+    std::mutex call_mu_, mu_;
+    std::condition_variable cv_, done_cv_;
+    std::vector<std::thread> th_;
+    std::vector<CopySeg> parts_;
+    size_t next_ = 0, done_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+CopyPool g_copy;
+
+void pin_ready(Workspace &w) {
+    if (w.pin[0]) return;
+    for (int b = 0; b < 2; b++) {
+        HIPCHK(hipHostMalloc((void **)&w.pin[b], PIN_CHUNK, hipHostMallocDefault));
+        HIPCHK(hipEventCreateWithFlags(&w.pin_ev[b], hipEventDisableTiming));
+    }
+}
+
+// The copies between the host spans, laid back to back, and pinned buffer `pin`, for bytes [o, o + len) of them.
+// `at` walks the spans (span index, offset in it) and moves on by len.
+std::vector<CopySeg> span_segs(const std::vector<Span> &sp, size_t len, uint8_t *pin, bool to_pin,
+                               std::pair<size_t, size_t> &at) {
+    std::vector<CopySeg> segs;
+    for (size_t done = 0; done < len;) {
+        const Span &x = sp[at.first];
+        const size_t take = std::min(len - done, x.n - at.second);
+        if (to_pin) segs.push_back({pin + done, x.p + at.second, take});
+        else segs.push_back({x.p + at.second, pin + done, take});
+        done += take;
+        at.second += take;
+        if (at.second == x.n) {
+            at.first++;
+            at.second = 0;
+        }
+    }
+    return segs;
+}
+
+size_t span_total(const std::vector<Span> &sp) {
+    size_t n = 0;
+    for (const Span &x : sp) n += x.n;
+    return n;
+}
+
+// Host spans to dst on stream s (returns once the last chunk is queued; the spans may then be reused).
+void to_device(Workspace &w, uint8_t *dst, const std::vector<Span> &sp, hipStream_t s) {
+    const size_t n = span_total(sp);
+    if (n < PIN_MIN) {
+        for (const Span &x : sp) {
+            HIPCHK(hipMemcpyAsync(dst, x.p, x.n, hipMemcpyHostToDevice, s));
+            dst += x.n;
+        }
+        return;
+    }
+    pin_ready(w);
+    std::pair<size_t, size_t> at{0, 0};
+    while (at.first < sp.size() && sp[at.first].n == 0) at.first++;
+    for (size_t o = 0, k = 0; o < n; o += PIN_CHUNK, k++) {
+        const size_t len = std::min(PIN_CHUNK, n - o);
+        const int b = (int)(k & 1);
+        HIPCHK(hipEventSynchronize(w.pin_ev[b]));   // the DMA that read this buffer last is done
+        g_copy.copy(span_segs(sp, len, w.pin[b], true, at));
+        HIPCHK(hipMemcpyAsync(dst + o, w.pin[b], len, hipMemcpyHostToDevice, s));
+        HIPCHK(hipEventRecord(w.pin_ev[b], s));
+    }
+}
+
+// src to the host spans on stream s, after the work queued on s so far (returns with the bytes in the spans).
+void to_host(Workspace &w, const std::vector<Span> &sp, const uint8_t *src, hipStream_t s) {
+    const size_t n = span_total(sp);
+    if (n < PIN_MIN) {
+        for (const Span &x : sp) {
+            HIPCHK(hipMemcpyAsync(x.p, src, x.n, hipMemcpyDeviceToHost, s));
+            src += x.n;
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        return;
+    }
+    pin_ready(w);
+    const size_t chunks = (n + PIN_CHUNK - 1) / PIN_CHUNK;
+    auto issue = [&](size_t k) {
+        const int b = (int)(k & 1);
+        HIPCHK(hipEventSynchronize(w.pin_ev[b]));
+        HIPCHK(hipMemcpyAsync(w.pin[b], src + k * PIN_CHUNK, std::min(PIN_CHUNK, n - k * PIN_CHUNK),
+                              hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(w.pin_ev[b], s));
+    };
+    std::pair<size_t, size_t> at{0, 0};
+    while (at.first < sp.size() && sp[at.first].n == 0) at.first++;
+    issue(0);
+    for (size_t k = 0; k < chunks; k++) {
+        if (k + 1 < chunks) issue(k + 1);   // the other buffer: its chunk was copied out in the step before
+        const int b = (int)(k & 1);
+        HIPCHK(hipEventSynchronize(w.pin_ev[b]));
+        g_copy.copy(span_segs(sp, std::min(PIN_CHUNK, n - k * PIN_CHUNK), w.pin[b], false, at));
+    }
 }
 
 uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi);
@@ -1086,15 +1271,14 @@ static void run_pending_batch(std::vector<Pending *> &b, bool fwd) {
         std::lock_guard<std::mutex> lk(w.mu);
         grow(w.T, (size_t)at + 64);
         if (fwd) grow(w.U, (size_t)at + 64);
-        for (int32_t k = 0; k < count; k++)
-            HIPCHK(hipMemcpyAsync(P<uint8_t>(w.T) + segs[k].offset, b[k]->T, (size_t)b[k]->n, hipMemcpyHostToDevice,
-                                  w.stream));
+        std::vector<Span> sp(count);
+        for (int32_t k = 0; k < count; k++) sp[k] = {const_cast<uint8_t *>(b[k]->T), (size_t)b[k]->n};
+        to_device(w, P<uint8_t>(w.T), sp, w.stream);
         uint8_t *out = fwd ? P<uint8_t>(w.U) : P<uint8_t>(w.T);
         if (fwd) forward_batch_dev(w, P<uint8_t>(w.T), out, segs.data(), count, I.data(), w.stream);
         else inverse_batch_dev(w, P<uint8_t>(w.T), out, segs.data(), count, I.data(), w.stream);
-        for (int32_t k = 0; k < count; k++)
-            HIPCHK(hipMemcpyAsync(b[k]->U, out + segs[k].offset, (size_t)b[k]->n, hipMemcpyDeviceToHost, w.stream));
-        HIPCHK(hipStreamSynchronize(w.stream));
+        for (int32_t k = 0; k < count; k++) sp[k].p = b[k]->U;
+        to_host(w, sp, out, w.stream);
     });
     const std::string err = g_last_error;
     for (int32_t k = 0, o = 0; k < count; o += (int32_t)nsamples(b[k]->n, b[k]->r), k++) {
@@ -1395,11 +1579,11 @@ int32_t kbwt_bwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs, i
         HIPCHK(hipMalloc(&in.p, (size_t)(hi - lo) + 64));
         HIPCHK(hipMalloc(&out.p, (size_t)(hi - lo) + 64));
         try {
-            HIPCHK(hipMemcpyAsync(in.p, T + lo, (size_t)(hi - lo), hipMemcpyHostToDevice, w.stream));
+            to_device(w, P<uint8_t>(in), {{const_cast<uint8_t *>(T) + lo, (size_t)(hi - lo)}}, w.stream);
             std::vector<kbwt_seg> g(segs, segs + count);
             for (auto &x : g) x.offset -= lo;
             forward_batch_dev(w, P<uint8_t>(in), P<uint8_t>(out), g.data(), count, I, w.stream);
-            HIPCHK(hipMemcpyAsync(U + lo, out.p, (size_t)(hi - lo), hipMemcpyDeviceToHost, w.stream));
+            to_host(w, {{U + lo, (size_t)(hi - lo)}}, P<uint8_t>(out), w.stream);
             HIPCHK(hipStreamSynchronize(w.stream));
         } catch (...) {
             (void)hipFree(in.p);
@@ -1422,11 +1606,11 @@ int32_t kbwt_unbwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs,
         Buf io;
         HIPCHK(hipMalloc(&io.p, (size_t)(hi - lo) + 64));
         try {
-            HIPCHK(hipMemcpyAsync(io.p, T + lo, (size_t)(hi - lo), hipMemcpyHostToDevice, w.stream));
+            to_device(w, P<uint8_t>(io), {{const_cast<uint8_t *>(T) + lo, (size_t)(hi - lo)}}, w.stream);
             std::vector<kbwt_seg> g(segs, segs + count);
             for (auto &x : g) x.offset -= lo;
             inverse_batch_dev(w, P<uint8_t>(io), P<uint8_t>(io), g.data(), count, I, w.stream);
-            HIPCHK(hipMemcpyAsync(U + lo, io.p, (size_t)(hi - lo), hipMemcpyDeviceToHost, w.stream));
+            to_host(w, {{U + lo, (size_t)(hi - lo)}}, P<uint8_t>(io), w.stream);
             HIPCHK(hipStreamSynchronize(w.stream));
         } catch (...) {
             (void)hipFree(io.p);
@@ -1515,6 +1699,14 @@ void kbwt_release(void) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;
         b->cap = 0;
+    }
+    for (int b = 0; b < 2; b++) {
+        if (!w.pin[b]) continue;
+        (void)hipEventSynchronize(w.pin_ev[b]);
+        (void)hipEventDestroy(w.pin_ev[b]);
+        (void)hipHostFree(w.pin[b]);
+        w.pin[b] = nullptr;
+        w.pin_ev[b] = nullptr;
     }
 }
 

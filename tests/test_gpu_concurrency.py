@@ -126,3 +126,35 @@ def test_gpu_concurrent_errors_stay_per_call(gpu):
         if k != 3:
             assert errs[k] is None
             assert np.array_equal(out[k][0], expect[k][0]) and np.array_equal(out[k][1], expect[k][1]), k
+
+
+def test_gpu_concurrent_large_calls(gpu):
+    """Concurrent calls whose batch holds more than a few MiB of host bytes: the blocks cross the boundaries of the
+    library's pinned staging chunks, in both directions, and every block still gets the oracle's result."""
+    kinds = ["markov2", "mixed", "dna", "logs", "random", "repeats"]
+    rng = np.random.default_rng(960)
+    sizes = [int(x) for x in rng.integers(1 << 20, 5 << 19, 12)]
+    sizes[0] = (1 << 21) + 1
+    blocks = [K.generate(kinds[i % len(kinds)], 960 + i, n) for i, n in enumerate(sizes)]
+    rs = [1 << K.codec18_rlog(T.size) for T in blocks]
+    expect = [refs.oracle_bwt(T, r) for T, r in zip(blocks, rs)]
+    say("concurrency: oracle done for", len(blocks), "blocks,", sum(sizes), "bytes")
+    got = [None] * len(blocks)
+    back = [None] * len(blocks)
+
+    def fwd(k):
+        def f():
+            got[k] = K.bwt_aux(blocks[k], rs[k])
+        return f
+
+    def inv(k):
+        def f():
+            eU, eI = expect[k]
+            back[k] = K.unbwt_aux(eU, rs[k], eI)
+        return f
+
+    run_threads([fwd(k) for k in range(len(blocks))])
+    run_threads([inv(k) for k in range(len(blocks))])
+    for k in range(len(blocks)):
+        assert np.array_equal(got[k][0], expect[k][0]) and np.array_equal(got[k][1], expect[k][1]), k
+        assert np.array_equal(back[k], blocks[k]), k
