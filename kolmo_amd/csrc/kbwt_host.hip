@@ -51,6 +51,8 @@ struct Buf {
     size_t cap = 0;
 };
 
+constexpr size_t LIST_PEEK = 1024;   // large-group list entries read back with the list counters
+
 struct Workspace {
     int dev = -1;
     hipStream_t stream = nullptr;
@@ -76,6 +78,7 @@ struct Workspace {
     size_t h_meta_cap = 0;
     unsigned long long *h_orand = nullptr, *h_orand_init = nullptr;
     uint32_t *h_m = nullptr;
+    uint32_t *h_peek = nullptr;             // 2 * LIST_PEEK: the first heads and sizes of a large-group list
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool ready = false;
     // per-kernel device time (HIP events around each launch, on the launch's stream) while profiling is on
@@ -157,6 +160,7 @@ Workspace &workspace() {
         HIPCHK(hipHostMalloc(&w.h_orand, 2 * sizeof(unsigned long long), hipHostMallocDefault));
         HIPCHK(hipHostMalloc(&w.h_orand_init, 2 * sizeof(unsigned long long), hipHostMallocDefault));
         HIPCHK(hipHostMalloc(&w.h_m, sizeof(uint32_t), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc(&w.h_peek, 2 * LIST_PEEK * sizeof(uint32_t), hipHostMallocDefault));
         w.h_orand_init[0] = 0ull;
         w.h_orand_init[1] = ~0ull;
         {   // the device epoch starts in the upper half of the epoch space
@@ -745,18 +749,30 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     uint32_t *gcnt = w.gcnt;          // [small groups, large groups] of the list being built
     const uint32_t rank_max = tune.seg_rank;
     std::vector<uint32_t> h_lh, h_ls, h_lo;
+    // The list counters, and the large list itself. While large groups remain, the first LIST_PEEK heads and sizes
+    // come back with the counters (one wait instead of two); a longer list takes a second copy.
+    bool peek = true;
     auto read_lists = [&](int c, uint32_t &Gs, uint32_t &Gl) {
+        const size_t np = peek ? std::min<size_t>(LIST_PEEK, std::min(w.lh[c].cap, w.ls[c].cap) / 4) : 0;
         HIPCHK(hipMemcpyAsync(w.h_gcnt, gcnt, 16, hipMemcpyDeviceToHost, s));
+        if (np) {
+            HIPCHK(hipMemcpyAsync(w.h_peek, w.lh[c].p, np * 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipMemcpyAsync(w.h_peek + LIST_PEEK, w.ls[c].p, np * 4, hipMemcpyDeviceToHost, s));
+        }
         HIPCHK(hipStreamSynchronize(s));
         Gs = w.h_gcnt[0];
         Gl = w.h_gcnt[1];
         h_lh.resize(Gl);
         h_ls.resize(Gl);
-        if (Gl) {
+        if (Gl && Gl <= np) {
+            memcpy(h_lh.data(), w.h_peek, (size_t)Gl * 4);
+            memcpy(h_ls.data(), w.h_peek + LIST_PEEK, (size_t)Gl * 4);
+        } else if (Gl) {
             HIPCHK(hipMemcpyAsync(h_lh.data(), w.lh[c].p, (size_t)Gl * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(h_ls.data(), w.ls[c].p, (size_t)Gl * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipStreamSynchronize(s));
         }
+        peek = Gl > 0;
     };
     auto zero_counts = [&] { launch_fill_u8(reinterpret_cast<uint8_t *>(gcnt), 0, 16, s); };
     int cur = 0, round = 0;
@@ -773,15 +789,44 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
             LAUNCHCHK();
         }
         uint32_t Es = 0, ml = 0;
-        // gather every key of the round before any rank changes
+        // every key of the round is gathered before any rank changes: the small groups' offsets and the large groups'
+        // keys first (neither writes a rank), then one wait for the small groups' rows and the digits that vary
         if (Gs) {
             TIMED("scan_groups", Gs,
                   launch_scan_groups(P<uint32_t>(w.gs[cur]), Gs, P<uint32_t>(w.go), w.m_out, P<uint32_t>(w.tstart),
                                      P<uint64_t>(w.st_sum), w.counters + 3, next_epoch(w), s));
             LAUNCHCHK();
             HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            Es = *w.h_m;
+        }
+        Shifts sh{};
+        if (Gl) {
+            // the large groups in increasing head order (regroup's head scan runs over SA positions), then their rows
+            std::vector<uint32_t> ord(Gl);
+            for (uint32_t g = 0; g < Gl; g++) ord[g] = g;
+            std::sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return h_lh[x] < h_lh[y]; });
+            h_lo.resize(2 * (size_t)Gl);
+            for (uint32_t k = 0; k < Gl; k++) {
+                h_lo[k] = h_lh[ord[k]];
+                h_lo[Gl + k] = ml;
+                ml += h_ls[ord[k]];
+            }
+            HIPCHK(hipMemcpyAsync(w.lo.p, h_lo.data(), (size_t)2 * Gl * 4, hipMemcpyHostToDevice, s));
+            TIMED("large_elems", ml,
+                  launch_large_elems(P<uint32_t>(w.lo), P<uint32_t>(w.lo) + Gl, Gl, ml, P<uint32_t>(w.pos[0]),
+                                     P<uint32_t>(w.grp[0]), s));
+            HIPCHK(hipMemcpyAsync(w.orand, w.h_orand_init, 16, hipMemcpyHostToDevice, s));
+            TIMED("round_keys", ml,
+                  launch_round_keys(P<uint32_t>(w.pos[0]), P<uint32_t>(w.grp[0]), ml, SA, ISA, N, blk, d_bl, (uint32_t)round,
+                                    K[0], V[0], w.orand, s));
+            LAUNCHCHK();
+            HIPCHK(hipMemcpyAsync(w.h_orand, w.orand, 16, hipMemcpyDeviceToHost, s));
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        if (Gs) Es = *w.h_m;
+        if (Gl) {
+            const uint64_t vary = w.h_orand[0] ^ w.h_orand[1];
+            for (uint32_t d = 0; d < 8; d++)
+                if ((vary >> (8 * d)) & 0xFFull) sh.s[sh.P++] = 8 * d;
         }
         SegRoundArgs sa{};
         sa.gh = P<uint32_t>(w.gh[cur]);
@@ -806,33 +851,6 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         sa.tmax = P<uint32_t>(w.tmax);
         sa.rank_max = rank_max;
         if (Gs) TIMED("seg_gather", Es, launch_seg_round(0, sa, s));
-        Shifts sh{};
-        if (Gl) {
-            // the large groups in increasing head order (regroup's head scan runs over SA positions), then their rows
-            std::vector<uint32_t> ord(Gl);
-            for (uint32_t g = 0; g < Gl; g++) ord[g] = g;
-            std::sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return h_lh[x] < h_lh[y]; });
-            h_lo.resize(2 * (size_t)Gl);
-            for (uint32_t k = 0; k < Gl; k++) {
-                h_lo[k] = h_lh[ord[k]];
-                h_lo[Gl + k] = ml;
-                ml += h_ls[ord[k]];
-            }
-            HIPCHK(hipMemcpyAsync(w.lo.p, h_lo.data(), (size_t)2 * Gl * 4, hipMemcpyHostToDevice, s));
-            TIMED("large_elems", ml,
-                  launch_large_elems(P<uint32_t>(w.lo), P<uint32_t>(w.lo) + Gl, Gl, ml, P<uint32_t>(w.pos[0]),
-                                     P<uint32_t>(w.grp[0]), s));
-            HIPCHK(hipMemcpyAsync(w.orand, w.h_orand_init, 16, hipMemcpyHostToDevice, s));
-            TIMED("round_keys", ml,
-                  launch_round_keys(P<uint32_t>(w.pos[0]), P<uint32_t>(w.grp[0]), ml, SA, ISA, N, blk, d_bl, (uint32_t)round,
-                                    K[0], V[0], w.orand, s));
-            LAUNCHCHK();
-            HIPCHK(hipMemcpyAsync(w.h_orand, w.orand, 16, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            const uint64_t vary = w.h_orand[0] ^ w.h_orand[1];
-            for (uint32_t d = 0; d < 8; d++)
-                if ((vary >> (8 * d)) & 0xFFull) sh.s[sh.P++] = 8 * d;
-        }
         // sort and split; the next round's lists are built from zero
         zero_counts();
         if (Gs) {
