@@ -20,8 +20,8 @@ measured with HIP events on its own stream inside the timed region (roofline), a
 The default cfg2 line also carries, under config: the batched form on cfg4 against one call per block ("batched"),
 the shipped router on the held-out router blocks ("router"), and the cfg2 block through the host-pointer entry
 points, PCIe included ("pcie_inclusive"), and the same block as 1 MiB blocks through those entry points from 16
-concurrent callers as kolmo's block workers make them, against one caller ("concurrent_calls"); --no-extras leaves
-them out.
+concurrent callers as kolmo's block workers make them, against one caller, from Python threads and from C threads
+("concurrent_calls"); --no-extras leaves them out.
 """
 import argparse
 import json
@@ -421,9 +421,15 @@ def concurrent_calls(K, workers=16, reps=3):
         res[name] = round(n / best / 2**20, 2)
     log(f"bench: 1 MiB blocks through kolmo_sais_*: {res['one_thread']} MiB/s from one thread, "
         f"{res['workers']} MiB/s from {workers} workers")
+    # the same calls from C threads (libkbwt_workers.so), with no interpreter lock between them
+    c1 = n / K.block_workers(T, 1 << 20, 1, reps) / 2**20
+    cw = n / K.block_workers(T, 1 << 20, workers, reps) / 2**20
+    log(f"bench: the same from C threads: {c1:.2f} MiB/s from one thread, {cw:.2f} MiB/s from {workers} workers")
     return {"blocks": len(blocks), "block_bytes": 1 << 20, "workers": workers, "unit": "MiB/s",
             "one_thread": res["one_thread"], "concurrent_workers": res["workers"],
-            "what": "kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux per block on host buffers (PCIe included), best of 3"}
+            "c_threads": {"one_thread": round(c1, 2), "concurrent_workers": round(cw, 2)},
+            "what": "kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux per block on host buffers (PCIe included), best of 3; "
+                    "from Python threads, and from C threads (c_threads)"}
 
 
 def router_summary():

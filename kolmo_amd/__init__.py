@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBDIR = os.path.join(HERE, "lib")
 HIP_LIB = os.path.join(LIBDIR, "libkbwt_hip.so")
 GEN_LIB = os.path.join(LIBDIR, "libkbwt_gen.so")
+WORKERS_LIB = os.path.join(LIBDIR, "libkbwt_workers.so")
 
 KBWT_OK, KBWT_ERR_ARG, KBWT_ERR_NOMEM, KBWT_ERR_DEVICE = 0, -1, -2, -3
 MAX_ROUNDS = 40
@@ -111,6 +112,38 @@ def lib():
         L.kbwt_dev_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
         _lib = L
     return _lib
+
+
+_workers = None
+
+
+def workers_lib():
+    """libkbwt_workers.so: kolmo's block workers as C threads over libkbwt_hip.so (loaded first)."""
+    global _workers
+    if _workers is None:
+        if not os.path.exists(WORKERS_LIB):
+            raise ImportError(f"{WORKERS_LIB} is missing: build it with `python -m kolmo_amd.build`")
+        lib()
+        W = ctypes.CDLL(WORKERS_LIB)
+        W.kbwt_workers_run.restype = ctypes.c_double
+        W.kbwt_workers_run.argtypes = [_u8p, ctypes.c_int64, ctypes.c_int32, _i32p, ctypes.c_int32, ctypes.c_int32]
+        _workers = W
+    return _workers
+
+
+def block_workers(T, block, workers, reps=1):
+    """Best seconds over `reps` passes of `workers` C threads over T cut into `block`-byte blocks, each thread taking
+    blocks in turn and calling kolmo_sais_bwt_aux then kolmo_sais_unbwt_aux (host buffers, codec 18's r per block),
+    every round trip checked against its block. Raises KbwtError when a call fails or a round trip differs."""
+    import numpy as np
+    T = np.ascontiguousarray(T, dtype=np.uint8)
+    sizes = [min(block, T.size - o) for o in range(0, T.size, block)]
+    rs = np.array([1 << codec18_rlog(m) for m in sizes], dtype=np.int32)
+    sec = workers_lib().kbwt_workers_run(_ptr(T), T.size, block, _ptr(rs, _i32p), workers, reps)
+    if sec < 0:
+        raise KbwtError(int(sec), "block workers: bad arguments" if sec == -1 else
+                        "block workers: a call failed or a round trip differs from its block")
+    return sec
 
 
 def gen_lib():

@@ -2,6 +2,7 @@
 
 - libkbwt_hip.so: the HIP kernels and the C-ABI of include/kbwt.h, compiled for gfx950 by hipcc.
 - libkbwt_gen.so: the seeded input generators (host C).
+- libkbwt_workers.so: kolmo's block workers as C threads over libkbwt_hip.so, for measuring concurrent calls.
 
 `python -m kolmo_amd.build` runs it; __graft_entry__.build() calls build_all().
 """
@@ -53,8 +54,19 @@ def build_gen(verbose=False):
     return out
 
 
+def build_workers(verbose=False):
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libkbwt_workers.so")
+    tmp = out + ".tmp"
+    _run(["gcc", "-O2", "-fPIC", "-shared", "-Wall", "-fvisibility=hidden", "-I", os.path.join(ROOT, "include"),
+          "-o", tmp, os.path.join(CSRC, "kbwt_workers.c"), "-L", LIB, "-lkbwt_hip", "-Wl,-rpath,$ORIGIN",
+          "-lpthread"], verbose)
+    os.replace(tmp, out)
+    return out
+
+
 def build_all(verbose=False):
-    return [build_hip(verbose), build_gen(verbose)]
+    return [build_hip(verbose), build_gen(verbose), build_workers(verbose)]
 
 
 if __name__ == "__main__":

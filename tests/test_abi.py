@@ -183,3 +183,20 @@ def test_concurrent_calls_without_device_fail_cleanly():
         t.join(timeout=60)
     assert not any(t.is_alive() for t in ts), "a caller is still waiting"
     assert len(codes) == 80 and set(codes) == {K.KBWT_ERR_DEVICE}
+
+
+def test_block_workers_harness_loads_and_checks_arguments():
+    """libkbwt_workers.so (kolmo's block workers as C threads) links against libkbwt_hip.so and rejects bad
+    arguments before any call; without a device a pass reports its failed calls (-2) instead of a time."""
+    W = K.workers_lib()
+    T = np.frombuffer(b"abracadabra" * 100, np.uint8).copy()
+    rs = np.array([64, 64], np.int32)
+    run = W.kbwt_workers_run
+    assert run(None, T.size, 600, K._ptr(rs, K._i32p), 2, 1) == -1.0
+    assert run(K._ptr(T), T.size, 1, K._ptr(rs, K._i32p), 2, 1) == -1.0          # blocks of one byte
+    assert run(K._ptr(T), T.size, 600, K._ptr(rs, K._i32p), 0, 1) == -1.0        # no workers
+    assert run(K._ptr(T), 1099, 1098, K._ptr(rs, K._i32p), 2, 1) == -1.0         # a last block of one byte
+    if not K.device_ok():
+        assert run(K._ptr(T), T.size, 600, K._ptr(rs, K._i32p), 2, 1) == -2.0
+        with pytest.raises(K.KbwtError):
+            K.block_workers(T, 600, 2)

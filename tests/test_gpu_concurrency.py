@@ -158,3 +158,14 @@ def test_gpu_concurrent_large_calls(gpu):
     for k in range(len(blocks)):
         assert np.array_equal(got[k][0], expect[k][0]) and np.array_equal(got[k][1], expect[k][1]), k
         assert np.array_equal(back[k], blocks[k]), k
+
+
+def test_gpu_block_workers_from_c_threads(gpu):
+    """kolmo's block workers as C threads (no interpreter lock between calls): 12 threads over 6.5 MiB in 1 MiB
+    blocks and a ragged last one, forward then inverse per block, every round trip equal to its block. One thread
+    over the same blocks as well."""
+    T = K.generate("mixed", 970, (13 << 19) + 333)
+    for workers in (12, 1):
+        sec = K.block_workers(T, 1 << 20, workers, reps=1)
+        say("block workers:", workers, "threads,", round(T.size / sec / 2**20, 1), "MiB/s")
+        assert sec > 0
