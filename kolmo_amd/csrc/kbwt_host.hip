@@ -180,7 +180,7 @@ Workspace &workspace() {
 // instead move their host spans (the blocks of a call or of a combined batch, back to back on the device) through two
 // pinned chunks: a small pool of CPU threads copies one chunk while the DMA engine moves the other.
 constexpr size_t PIN_CHUNK = (size_t)8 << 20;
-constexpr size_t PIN_MIN = (size_t)4 << 20;   // below this many bytes in all the runtime's own path is as fast
+constexpr size_t PIN_MIN = (size_t)1 << 20;   // below this many bytes in all the runtime's own path is as fast
 
 struct Span {   // host bytes
     uint8_t *p;
