@@ -1,6 +1,7 @@
 // Host side of libkbwt_hip: the C-ABI entry points of include/kbwt.h, the per-device workspace, and the steps of the
 // forward suffix sort (round loop) and of the inverse, for a batch of blocks (one block is the batch of one);
-// concurrent kolmo_sais_* calls are gathered into batches; the per-size-class tuning table and the autotuner.
+// concurrent kolmo_sais_* calls are gathered into batches; host buffers move through pinned double buffers; the
+// per-size-class tuning table and the autotuner.
 // Validation and return codes follow src/algo/sais/sais.c:12027-12082 of kolmo 0.0.2.
 #include "kbwt_internal.h"
 #include "kbwt_router.h"
