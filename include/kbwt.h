@@ -7,12 +7,17 @@
  * codec-5 and codec-18 stream byte-identical. The output never depends on `threads`, as sais.h:5 requires.
  *
  * Return codes: 0 ok; -1 bad arguments or indexes out of range (sais.c:12029-12032, 12057-12066); -2 out of memory
- * (host or device); -3 a HIP runtime error (no usable device, a failed launch). There is no CPU fallback inside this
- * library: a device error is reported, on stderr and through kbwt_last_error(), never hidden.
+ * (host or device), or a block longer than 2^31 - 65 bytes, which the device kernels cannot index; -3 a HIP runtime
+ * error (no usable device, a failed launch). A device error is reported on stderr and through kbwt_last_error(), and
+ * the library recovers from it (the stream is drained and the cached state reset) before the call returns, so the
+ * next call is unaffected. The CPU path is off by default: with kbwt_set_cpu_fallback(1) (or KBWT_CPU_FALLBACK=1) a
+ * host-pointer call whose device work fails is computed on the CPU instead, with identical bytes, as kolmo's Metal lane
+ * did (src/arch/metal/cubwt_metal.c:4-7), and counted as kolmo counts GPU fallbacks (src/core/hetero.c:62-66).
  *
  * Every function is reentrant; calls from several host threads are serialised per device, except that concurrent
  * kolmo_sais_* calls (kolmo's block workers) are gathered into batches: one caller runs every block queued at that
- * moment as one kbwt batch, and each block still gets exactly its single-block result. */
+ * moment as one kbwt batch, and each block still gets exactly its single-block result. When such a batch fails, each
+ * of its blocks runs again on its own, so a failure stays with the call it belongs to. */
 #ifndef KBWT_H
 #define KBWT_H
 #include <stddef.h>
@@ -163,6 +168,28 @@ KBWT_API void kbwt_profile(int32_t on);
 KBWT_API void kbwt_profile_only(const char *name);
 KBWT_API int32_t kbwt_profile_read(kbwt_kernel_time *out, int32_t max);
 KBWT_API int32_t kbwt_dev_copy(void *dst, const void *src, size_t bytes, void *stream);
+
+/* ---- error path (no counterpart in the reference's sais.h; it mirrors kolmo's GPU worker, src/core/hetero.c) ----
+   kbwt_set_cpu_fallback(on): host-pointer calls (kolmo_sais_*, kbwt_bwt_aux_batch, kbwt_unbwt_aux_batch) whose device
+   work fails with -2 or -3 are computed by the library's CPU path instead (src/arch/metal/cubwt_metal.c:4-7). Off by
+   default; KBWT_CPU_FALLBACK=1 in the environment turns it on. Returns the previous setting. The device-pointer
+   entry points never fall back.
+   kbwt_fault_inject(spec): host-side fault injection for tests. spec "site:k" or "site:k:count", site one of alloc,
+   launch, sync, copy: occurrences k .. k+count-1 of that fault point, counted from this call, fail as if the HIP
+   runtime had returned an error (alloc: -2, the others: -3), before the runtime is called. NULL or "" disarms it.
+   KBWT_FAULT in the environment arms it at load. Returns -1 for a spec it cannot parse.
+   kbwt_get_counters: process-wide counts since load (kolmo's KOLMO_GPU_STATS, src/core/hetero.c:19-22,38). */
+typedef struct {
+    int64_t device_errors;     /* calls whose device work failed (-2 or -3 from the device path) */
+    int64_t retried_blocks;    /* blocks of a failed combined batch that ran again on their own */
+    int64_t cpu_fallbacks;     /* blocks computed by the CPU path after a device error */
+    int64_t injected_faults;   /* faults injected by kbwt_fault_inject / KBWT_FAULT */
+    int64_t combined_batches;  /* batches of concurrent kolmo_sais_* calls */
+    int64_t combined_blocks;   /* blocks those batches held */
+} kbwt_counters;
+KBWT_API int32_t kbwt_set_cpu_fallback(int32_t on);
+KBWT_API int32_t kbwt_fault_inject(const char *spec);
+KBWT_API int32_t kbwt_get_counters(kbwt_counters *out);
 
 /* Message of the calling thread's last error ("" when none). */
 KBWT_API const char *kbwt_last_error(void);

@@ -5,8 +5,9 @@ for Python callers and tests. The raw entry points keep the names, argument mean
 src/algo/sais/sais.h:13,17,20 of kolmo 0.0.2 (kolmo_sais_bwt_aux, kolmo_sais_unbwt_aux, kolmo_sais_unbwt); the
 convenience wrappers bwt_aux / unbwt_aux raise KbwtError on a non-zero code.
 
-There is no CPU path here: loading fails loudly when the library is missing, and a call fails with code -3 when no
-usable device is present.
+There is no CPU path in this module: loading fails loudly when the library is missing, and a call fails with code -3
+when no usable device is present. The library's own CPU path after a device error is opt-in
+(set_cpu_fallback(True), or KBWT_CPU_FALLBACK=1) and every use of it is counted (counters()).
 """
 import ctypes
 import os
@@ -48,6 +49,16 @@ class Stats(ctypes.Structure):
 class KernelTime(ctypes.Structure):
     _fields_ = [("name", ctypes.c_char * 32), ("ms", ctypes.c_double), ("launches", ctypes.c_int64),
                 ("elems", ctypes.c_int64)]
+
+
+class Counters(ctypes.Structure):
+    """kbwt_counters: process-wide counts of the error path and of combined batches."""
+    _fields_ = [("device_errors", ctypes.c_int64), ("retried_blocks", ctypes.c_int64),
+                ("cpu_fallbacks", ctypes.c_int64), ("injected_faults", ctypes.c_int64),
+                ("combined_batches", ctypes.c_int64), ("combined_blocks", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 class Seg(ctypes.Structure):
@@ -108,6 +119,12 @@ def lib():
             f.restype = ctypes.c_int32
             f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Seg), ctypes.c_int32, _i32p,
                           ctypes.c_void_p]
+        L.kbwt_set_cpu_fallback.restype = ctypes.c_int32
+        L.kbwt_set_cpu_fallback.argtypes = [ctypes.c_int32]
+        L.kbwt_fault_inject.restype = ctypes.c_int32
+        L.kbwt_fault_inject.argtypes = [ctypes.c_char_p]
+        L.kbwt_get_counters.restype = ctypes.c_int32
+        L.kbwt_get_counters.argtypes = [ctypes.POINTER(Counters)]
         L.kbwt_dev_copy.restype = ctypes.c_int32
         L.kbwt_dev_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
         _lib = L
@@ -285,6 +302,27 @@ def last_stats():
 
 def device_ok():
     return bool(lib().kbwt_device_ok())
+
+
+def counters():
+    """kbwt_get_counters: device errors, blocks retried alone, CPU-path fallbacks, injected faults, combined batches."""
+    c = Counters()
+    _check(lib().kbwt_get_counters(ctypes.byref(c)))
+    return c.as_dict()
+
+
+def set_cpu_fallback(on):
+    """kbwt_set_cpu_fallback: the library's CPU path for host-pointer calls whose device work fails. Returns the
+    previous setting."""
+    return bool(lib().kbwt_set_cpu_fallback(1 if on else 0))
+
+
+def fault_inject(spec):
+    """kbwt_fault_inject: 'site:k[:count]' (site alloc, launch, sync or copy) fails those occurrences of the fault
+    point as if the HIP runtime had; None or '' disarms."""
+    rc = lib().kbwt_fault_inject(spec.encode() if spec else None)
+    if rc != 0:
+        raise KbwtError(rc, f"bad fault spec {spec!r}")
 
 
 # ---- sampling-interval rules of the callers (src/codecs/bwt_stream.c:40-44 and :224-228) ----

@@ -4,11 +4,15 @@
 // per-size-class tuning table and the autotuner.
 // Validation and return codes follow src/algo/sais/sais.c:12027-12082 of kolmo 0.0.2.
 #include "kbwt_internal.h"
+#include "kbwt_combine.h"
+#include "kbwt_copypool.h"
+#include "kbwt_cpu.h"
 #include "kbwt_router.h"
 #include "../../include/kbwt.h"
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
@@ -45,7 +49,81 @@ struct DevError {
         hipError_t e_ = (x);                           \
         if (e_ != hipSuccess) fail(e_, #x);            \
     } while (0)
-#define LAUNCHCHK() HIPCHK(hipGetLastError())
+
+// ---- counters of the error path (kbwt_get_counters), and host-side fault injection
+//
+// A fault point is a place where the runtime can report an error: a device allocation, a kernel launch, a stream
+// synchronisation, a host <-> device copy. kbwt_fault_inject("site:k[:count]") (or KBWT_FAULT in the environment at
+// load) makes occurrences k .. k+count-1 of that site, counted from the call that armed it, fail before the runtime is
+// called, as if it had returned an error. Nothing is launched differently: every kernel queued before the fault
+// completes normally, and the error path (drain the stream, reset the cached state, run failed blocks again on their
+// own, the optional CPU path) is exercised without a fault on the device.
+enum FaultSite { FS_ALLOC, FS_LAUNCH, FS_SYNC, FS_COPY, FS_N };
+const char *const FAULT_NAMES[FS_N] = {"alloc", "launch", "sync", "copy"};
+struct FaultPlan {
+    std::atomic<int> site{-1};
+    std::atomic<uint64_t> at{0}, count{0}, seen{0};
+};
+FaultPlan g_fault;
+struct Counters {
+    std::atomic<int64_t> device_errors{0}, retried_blocks{0}, cpu_fallbacks{0}, injected_faults{0};
+};
+Counters g_counters;
+std::atomic<int> g_cpu_fallback{-1};   // -1: not read yet from KBWT_CPU_FALLBACK
+
+bool parse_fault(const char *spec, int &site, uint64_t &at, uint64_t &count) {
+    site = -1;
+    if (!spec || !*spec) return true;   // disarm
+    const char *colon = strchr(spec, ':');
+    if (!colon) return false;
+    for (int k = 0; k < FS_N; k++)
+        if ((size_t)(colon - spec) == strlen(FAULT_NAMES[k]) && strncmp(spec, FAULT_NAMES[k], colon - spec) == 0)
+            site = k;
+    if (site < 0) return false;
+    char *end = nullptr;
+    at = strtoull(colon + 1, &end, 10);
+    count = 1;
+    if (*end == ':') count = strtoull(end + 1, &end, 10);
+    if (*end != 0 || at == 0 || count == 0) {
+        site = -1;
+        return false;
+    }
+    return true;
+}
+void arm_fault(int site, uint64_t at, uint64_t count) {
+    g_fault.site.store(-1);
+    g_fault.at.store(at);
+    g_fault.count.store(count);
+    g_fault.seen.store(0);
+    g_fault.site.store(site);
+}
+struct FaultFromEnv {
+    FaultFromEnv() {
+        int site;
+        uint64_t at, count;
+        if (parse_fault(getenv("KBWT_FAULT"), site, at, count) && site >= 0) arm_fault(site, at, count);
+    }
+} g_fault_env;
+
+void fault_point(FaultSite site) {
+    if (g_fault.site.load(std::memory_order_relaxed) != (int)site) return;
+    const uint64_t k = g_fault.seen.fetch_add(1) + 1, at = g_fault.at.load();
+    if (k < at || k - at >= g_fault.count.load()) return;
+    g_counters.injected_faults++;
+    char what[64];
+    snprintf(what, sizeof what, "%s (injected fault %llu)", FAULT_NAMES[site], (unsigned long long)k);
+    fail(site == FS_ALLOC ? hipErrorOutOfMemory : site == FS_LAUNCH ? hipErrorLaunchFailure : hipErrorUnknown, what);
+}
+#define LAUNCHCHK()                        \
+    do {                                   \
+        fault_point(FS_LAUNCH);            \
+        HIPCHK(hipGetLastError());         \
+    } while (0)
+#define SYNCCHK(st)                              \
+    do {                                         \
+        fault_point(FS_SYNC);                    \
+        HIPCHK(hipStreamSynchronize(st));        \
+    } while (0)
 
 struct Buf {
     void *p = nullptr;
@@ -59,6 +137,7 @@ struct Workspace {
     hipStream_t stream = nullptr;
     std::mutex mu;
     uint32_t epoch = 1;
+    uint32_t dev_epoch_calls = 0;           // inverse calls since the device epoch last started a cycle
     Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, carry, samples, stage, blk, counts, dbase,
         tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es,
         ecnt, eoff, pair, tmax, lrow;
@@ -122,12 +201,18 @@ void grow(Buf &b, size_t bytes, bool zero = false) {
     b.p = nullptr;
     b.cap = 0;
     size_t cap = bytes + bytes / 8;   // room to grow a little without a new allocation
+    fault_point(FS_ALLOC);
     HIPCHK(hipMalloc(&b.p, cap));
     if (zero) {
         HIPCHK(hipMemset(b.p, 0, cap));
         HIPCHK(hipDeviceSynchronize());
     }
     b.cap = cap;
+}
+
+void clear_status(Workspace &w, hipStream_t s) {
+    if (w.status.p) launch_fill_u8(reinterpret_cast<uint8_t *>(w.status.p), 0, w.status.cap, s);
+    if (w.st_sum.p) launch_fill_u8(reinterpret_cast<uint8_t *>(w.st_sum.p), 0, w.st_sum.cap, s);
 }
 
 void *host_staging(Workspace &w, size_t bytes) {
@@ -187,86 +272,6 @@ struct Span {   // host bytes
     uint8_t *p;
     size_t n;
 };
-struct CopySeg {
-    uint8_t *dst;
-    const uint8_t *src;
-    size_t n;
-};
-
-// memcpy of a list of segments split over a few threads (the caller takes part); one list at a time.
-class CopyPool {
-public:
-    void copy(const std::vector<CopySeg> &segs) {
-        constexpr size_t PART = (size_t)512 << 10;
-        std::lock_guard<std::mutex> one(call_mu_);
-        parts_.clear();
-        for (const CopySeg &g : segs)
-            for (size_t o = 0; o < g.n; o += PART) parts_.push_back({g.dst + o, g.src + o, std::min(PART, g.n - o)});
-        if (parts_.size() < 2) {
-            for (const CopySeg &g : parts_) memcpy(g.dst, g.src, g.n);
-            return;
-        }
-        start();
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            next_ = 0;
-            done_ = 0;
-            gen_++;
-        }
-        cv_.notify_all();
-        work();
-        std::unique_lock<std::mutex> lk(mu_);
-        done_cv_.wait(lk, [&] { return done_ == parts_.size(); });
-    }
-    ~CopyPool() {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
-        }
-        cv_.notify_all();
-        for (auto &t : th_) t.join();
-    }
-
-private:
-    void start() {   // under call_mu_
-        if (!th_.empty()) return;
-        const unsigned hw = std::thread::hardware_concurrency();
-        const unsigned k = hw > 2 ? std::min(7u, hw - 1) : 1u;
-        for (unsigned i = 0; i < k; i++) th_.emplace_back([this] { loop(); });
-    }
-    void loop() {
-        uint64_t seen = 0;
-        for (;;) {
-            {
-                std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
-                seen = gen_;
-            }
-            work();
-        }
-    }
-    void work() {   // take parts until none is left
-        for (;;) {
-            size_t k;
-            {
-                std::lock_guard<std::mutex> lk(mu_);
-                if (next_ >= parts_.size()) return;
-                k = next_++;
-            }
-            memcpy(parts_[k].dst, parts_[k].src, parts_[k].n);
-            std::lock_guard<std::mutex> lk(mu_);
-            if (++done_ == parts_.size()) done_cv_.notify_all();
-        }
-    }
-    std::mutex call_mu_, mu_;
-    std::condition_variable cv_, done_cv_;
-    std::vector<std::thread> th_;
-    std::vector<CopySeg> parts_;
-    size_t next_ = 0, done_ = 0;
-    uint64_t gen_ = 0;
-    bool stop_ = false;
-};
 CopyPool g_copy;
 
 void pin_ready(Workspace &w) {
@@ -305,6 +310,7 @@ size_t span_total(const std::vector<Span> &sp) {
 
 // Host spans to dst on stream s (returns once the last chunk is queued; the spans may then be reused).
 void to_device(Workspace &w, uint8_t *dst, const std::vector<Span> &sp, hipStream_t s) {
+    fault_point(FS_COPY);
     const size_t n = span_total(sp);
     if (n < PIN_MIN) {
         for (const Span &x : sp) {
@@ -328,13 +334,14 @@ void to_device(Workspace &w, uint8_t *dst, const std::vector<Span> &sp, hipStrea
 
 // src to the host spans on stream s, after the work queued on s so far (returns with the bytes in the spans).
 void to_host(Workspace &w, const std::vector<Span> &sp, const uint8_t *src, hipStream_t s) {
+    fault_point(FS_COPY);
     const size_t n = span_total(sp);
     if (n < PIN_MIN) {
         for (const Span &x : sp) {
             HIPCHK(hipMemcpyAsync(x.p, src, x.n, hipMemcpyDeviceToHost, s));
             src += x.n;
         }
-        HIPCHK(hipStreamSynchronize(s));
+        SYNCCHK(s);
         return;
     }
     pin_ready(w);
@@ -359,11 +366,16 @@ void to_host(Workspace &w, const std::vector<Span> &sp, const uint8_t *src, hipS
 
 uint32_t env_u32(const char *name, uint32_t dflt, uint32_t lo, uint32_t hi);
 
-// Host-side lookback epochs stay in [1, EPOCH_HALF); graphs use the upper half.
-inline uint32_t next_epoch(Workspace &w) {
-    const uint32_t e = w.epoch;
-    w.epoch = w.epoch + 1 < EPOCH_HALF ? w.epoch + 1 : 1u;
-    return e;
+// Lookback status words are never cleared between launches: an epoch tag tells a word of this launch from older
+// ones. Host-side epochs stay in [1, EPOCH_HALF - 1); before they wrap, the status arrays are cleared (a fill kernel,
+// ordered on the stream like the launches), so a word left by a launch of the previous cycle cannot read as ready.
+void clear_status(Workspace &w, hipStream_t s);
+inline uint32_t next_epoch(Workspace &w, hipStream_t s) {
+    if (w.epoch >= EPOCH_HALF - 1) {
+        clear_status(w, s);
+        w.epoch = 1;
+    }
+    return w.epoch++;
 }
 
 // Whether any of the named launches would be bracketed by profiling events.
@@ -394,7 +406,12 @@ void run_graph(G &g, const std::vector<uint64_t> &key, hipStream_t s, std::initi
         }
         hipGraph_t graph = nullptr;
         HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        body();
+        try {
+            body();
+        } catch (...) {   // leave the stream out of capture mode before the error goes on
+            if (hipStreamEndCapture(s, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
         HIPCHK(hipStreamEndCapture(s, &graph));
         const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -502,6 +519,10 @@ uint32_t size_class(uint32_t N) {
     while (c < 32 && ((uint64_t)1 << c) < N) c++;
     return c;
 }
+Tuning tuning_for_class(uint32_t cls) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return g_tune[cls];
+}
 Tuning tuning_for(uint32_t N) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     Tuning t = g_tune[size_class(N)];
@@ -544,13 +565,13 @@ void router_features_tail(Workspace &w, const uint8_t *dT, const BlockInfo *d_bl
         launch_order1(dT, tl, first[nb], d_bl + b0, nb, P<uint32_t>(w.bigram), P<float>(w.rfeat) + (size_t)b0 * ROUTER_NF,
                       s);
         LAUNCHCHK();
-        HIPCHK(hipStreamSynchronize(s));   // rmeta and first are reused by the next group
+        SYNCCHK(s);   // rmeta and first are reused by the next group
     }
     std::vector<float> ent((size_t)B * ROUTER_NF);
     std::vector<uint32_t> act((size_t)ROUTER_ROUNDS * B);
     HIPCHK(hipMemcpyAsync(ent.data(), w.rfeat.p, ent.size() * 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(act.data(), d_act, act.size() * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    SYNCCHK(s);
     for (uint32_t b = 0; b < B; b++) {
         float *f = feat + (size_t)b * ROUTER_NF;
         f[256] = ent[(size_t)b * ROUTER_NF + 256];
@@ -631,7 +652,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
           launch_scan_digits(P<uint32_t>(w.counts), B, 1, d_bl, 0, P<uint32_t>(w.tot), nullptr, s));
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(h_counts, w.tot.p, (size_t)B * 256 * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    SYNCCHK(s);
     // 56 key bits above the BWT byte: 8 symbols when a symbol needs 5-7 bits, 7 bytes of binary data, and as many
     // symbols as fit for small alphabets (14 of 4 bits, 18 of 3, 28 of 2, 56 of 1)
     uint32_t P0 = 1;
@@ -691,7 +712,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         a.tiling = tsort;
         a.status = P<uint64_t>(w.status);
         a.counter = w.counters + 0;
-        a.epoch = next_epoch(w);
+        a.epoch = next_epoch(w, s);
         a.T = dT;
         a.codes = d_codes;
         TIMED(p == 0 ? "sort_pass_T" : "sort_pass", N, launch_sort_pass(a, nst, s));
@@ -728,7 +749,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         // then write window by window instead of scattering 4-byte writes over all of ISA
         TIMED("isa_part", N,
               launch_partition_pairs(K[P0 & 1], K[(P0 - 1) & 1], N, tpart, npt, P<uint64_t>(w.status), w.counters + 4,
-                                     next_epoch(w), s));
+                                     next_epoch(w, s), s));
         LAUNCHCHK();
         TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, ISA, s));
     }
@@ -760,7 +781,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
             HIPCHK(hipMemcpyAsync(w.h_peek, w.lh[c].p, np * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(w.h_peek + LIST_PEEK, w.ls[c].p, np * 4, hipMemcpyDeviceToHost, s));
         }
-        HIPCHK(hipStreamSynchronize(s));
+        SYNCCHK(s);
         Gs = w.h_gcnt[0];
         Gl = w.h_gcnt[1];
         h_lh.resize(Gl);
@@ -771,7 +792,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         } else if (Gl) {
             HIPCHK(hipMemcpyAsync(h_lh.data(), w.lh[c].p, (size_t)Gl * 4, hipMemcpyDeviceToHost, s));
             HIPCHK(hipMemcpyAsync(h_ls.data(), w.ls[c].p, (size_t)Gl * 4, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
+            SYNCCHK(s);
         }
         peek = Gl > 0;
     };
@@ -795,7 +816,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         if (Gs) {
             TIMED("scan_groups", Gs,
                   launch_scan_groups(P<uint32_t>(w.gs[cur]), Gs, P<uint32_t>(w.go), w.m_out, P<uint32_t>(w.tstart),
-                                     P<uint64_t>(w.st_sum), w.counters + 3, next_epoch(w), s));
+                                     P<uint64_t>(w.st_sum), w.counters + 3, next_epoch(w, s), s));
             LAUNCHCHK();
             HIPCHK(hipMemcpyAsync(w.h_m, w.m_out, 4, hipMemcpyDeviceToHost, s));
         }
@@ -822,7 +843,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
             LAUNCHCHK();
             HIPCHK(hipMemcpyAsync(w.h_orand, w.orand, 16, hipMemcpyDeviceToHost, s));
         }
-        HIPCHK(hipStreamSynchronize(s));
+        SYNCCHK(s);
         if (Gs) Es = *w.h_m;
         if (Gl) {
             const uint64_t vary = w.h_orand[0] ^ w.h_orand[1];
@@ -862,7 +883,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
             const uint32_t nt = (Es + SEG_TILE - 1) / SEG_TILE;
             TIMED("emit_compact", Es,
                   launch_scan_groups(P<uint32_t>(w.ecnt), nt, P<uint32_t>(w.eoff), gcnt, nullptr,
-                                     P<uint64_t>(w.st_sum), w.counters + 3, next_epoch(w), s);
+                                     P<uint64_t>(w.st_sum), w.counters + 3, next_epoch(w, s), s);
                   launch_compact_emits(P<uint32_t>(w.eh), P<uint32_t>(w.es), P<uint32_t>(w.ecnt), P<uint32_t>(w.eoff),
                                        nt, P<uint32_t>(w.gh[cur ^ 1]), P<uint32_t>(w.gs[cur ^ 1]), s));
         }
@@ -891,7 +912,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
                     a.tiling = flat;
                     a.status = P<uint64_t>(w.status);
                     a.counter = w.counters + 0;
-                    a.epoch = next_epoch(w);
+                    a.epoch = next_epoch(w, s);
                     TIMED("sort_pass", ml, launch_sort_pass(a, ntl, s));
                     LAUNCHCHK();
                     src ^= 1;
@@ -932,7 +953,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     LAUNCHCHK();
     HIPCHK(hipEventRecord(w.ev1, s));
     HIPCHK(hipMemcpyAsync(I, w.samples.p, (size_t)nsamp * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    SYNCCHK(s);
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, w.ev0, w.ev1));
     st.ms_fwd = ms;
@@ -946,6 +967,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
 void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockInfo> &bl, uint32_t N, hipStream_t s) {
     const uint32_t B = (uint32_t)bl.size();
     kbwt_stats &st = g_stats;
+    memset(&st, 0, sizeof st);   // the stats describe this call alone: the forward's fields stay zero
     st.n = (int32_t)N;
     st.blocks = (int32_t)B;
     // one splitter row in D = 2^dshift; the first cap bytes of each sublist are staged during the counting walk
@@ -1014,6 +1036,12 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
         (uintptr_t)w.V0.p, (uintptr_t)w.K0.p, (uintptr_t)w.K1.p, (uintptr_t)w.pos[0].p, (uintptr_t)w.pos[1].p,
         (uintptr_t)w.stage.p, (uintptr_t)w.status.p, (uintptr_t)w.counts.p, (uintptr_t)w.cbase.p,
         (uintptr_t)w.bad.p, (uintptr_t)w.blk.p, (uintptr_t)w.counters, (uintptr_t)w.pair.p};
+    // every call moves the device epoch on by EPOCH_STRIDE inside [EPOCH_HALF, 2 * EPOCH_HALF); before it wraps, the
+    // status words are cleared (outside any graph)
+    if (++w.dev_epoch_calls >= EPOCH_HALF / EPOCH_STRIDE - 1) {
+        clear_status(w, s);
+        w.dev_epoch_calls = 0;
+    }
     run_graph(w.inv_graph, key, s,
               {"fill_blk", "byte_hist", "scan_digits", "lf_pass", "lf_pairs", "walk_count", "rank_jump", "walk_emit"}, [&] {
         launch_epoch_bump(depoch, s);
@@ -1041,7 +1069,7 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     });
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(h_bad, w.bad.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    SYNCCHK(s);
     bool anybad = false;
     for (uint32_t b = 0; b < B; b++) anybad |= h_bad[b] != 0;
     if (anybad) {   // strings that are no BWT decode to zeros: deterministic bytes that fail the caller's checksum
@@ -1049,7 +1077,7 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
         LAUNCHCHK();
     }
     HIPCHK(hipEventRecord(w.ev1, s));
-    HIPCHK(hipStreamSynchronize(s));
+    SYNCCHK(s);
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, w.ev0, w.ev1));
     st.ms_inv = ms;
@@ -1143,14 +1171,14 @@ void forward_batch_dev(Workspace &w, const uint8_t *dT, uint8_t *dU, const kbwt_
                 if (segs[b].n == 1) {
                     uint8_t c = 0;
                     HIPCHK(hipMemcpyAsync(&c, dT + segs[b].offset, 1, hipMemcpyDeviceToHost, s));
-                    HIPCHK(hipStreamSynchronize(s));
+                    SYNCCHK(s);
                     f[c] = 1.f;
                 }
             }
         }
     for (size_t k = 0; k < bl.size(); k++)
         memcpy(I + soff[which[k]], Id.data() + bl[k].soff, nsamples((int32_t)bl[k].n, (int32_t)bl[k].r) * 4);
-    HIPCHK(hipStreamSynchronize(s));
+    SYNCCHK(s);
 }
 
 // Inverse on a batch in device memory; dT == dU allowed.
@@ -1193,7 +1221,7 @@ void inverse_batch_dev(Workspace &w, const uint8_t *dT, uint8_t *dU, const kbwt_
     for (int32_t b = 0; b < count; b++)
         if (segs[b].n == 1)
             HIPCHK(hipMemcpyAsync(dU + segs[b].offset, dT + segs[b].offset, 1, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
+    SYNCCHK(s);
 }
 
 bool samples_ok(const kbwt_seg *segs, int32_t count, const int32_t *I) {
@@ -1207,6 +1235,35 @@ bool samples_ok(const kbwt_seg *segs, int32_t count, const int32_t *I) {
             if (I[k + t] <= 0 || I[k + t] > n) return false;
     }
     return true;
+}
+
+// After an error inside a call: leave capture mode if the error came during a capture, let every launch queued so far
+// finish (each is a whole, valid launch, so each ends), drop the cached inverse graph (its buffers may have been
+// freed by a failed allocation) and the profiling events of unfinished launches, and clear the runtime's last error.
+// The next call then starts from a clean workspace. Never throws.
+void recover(Workspace &w, hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    }
+    (void)hipStreamSynchronize(s);
+    if (s != w.stream) (void)hipStreamSynchronize(w.stream);
+    for (int b = 0; b < 2; b++)
+        if (w.pin_ev[b]) (void)hipEventSynchronize(w.pin_ev[b]);
+    if (w.inv_graph.exec) (void)hipGraphExecDestroy(w.inv_graph.exec);
+    w.inv_graph.exec = nullptr;
+    w.inv_graph.key.clear();
+    w.inv_graph.last.clear();
+    try {
+        for (auto &p : w.pending) {
+            w.spare.push_back(p.a);
+            w.spare.push_back(p.b);
+        }
+    } catch (...) {
+    }
+    w.pending.clear();
+    (void)hipGetLastError();
 }
 
 }  // namespace
@@ -1226,6 +1283,7 @@ template <class F> static int32_t guarded(F f) {
         f();
         return 0;
     } catch (const DevError &e) {
+        g_counters.device_errors++;
         return e.code;
     } catch (const std::bad_alloc &) {
         g_last_error = "host allocation failed";
@@ -1233,43 +1291,52 @@ template <class F> static int32_t guarded(F f) {
     }
 }
 
-// ---- concurrent single-block calls run as batches
-//
-// kolmo's block workers call kolmo_sais_bwt_aux and kolmo_sais_unbwt_aux concurrently (src/core/compress.c:231-256,
-// src/core/decode.c:404-410). A call queues its block; a caller that finds no batch running takes every queued block
-// (up to the batch limits), runs the forward blocks as one batch and the inverse blocks as another, and wakes the
-// others. Each block's result is the single-block result, so batching changes nothing but the launch count.
-struct Pending {
-    bool fwd;
-    const uint8_t *T;
-    uint8_t *U;
-    int32_t n, r;        // r: the sampling interval (forward) or r of the inverse
-    int32_t *I;          // forward: the sampled indexes out
-    const int32_t *Iin;  // inverse: the sampled indexes in
-    int32_t rc = 0;
-    bool done = false;
-    std::string err;
-};
-static std::mutex g_comb_mu;
-static std::condition_variable g_comb_cv;
-static std::vector<Pending *> g_comb_q;
-static bool g_comb_busy = false;
-
-// One batch of one kind from host buffers: blocks packed back to back in the workspace's staging buffers.
-static void run_pending_batch(std::vector<Pending *> &b, bool fwd);
-// Never throws: a caller that runs a batch must always wake the others.
-static void run_pending(std::vector<Pending *> &b, bool fwd) {
-    try {
-        run_pending_batch(b, fwd);
-    } catch (...) {
-        for (Pending *q : b) {
-            q->rc = KBWT_ERR_NOMEM;
-            q->err = "host allocation failed";
+// f(w, s) on the current device's workspace, which it holds locked; s is the caller's stream, or the library's. On
+// an error the workspace recovers (recover) before the code goes back to the caller, so the next call is unaffected.
+template <class F> static int32_t on_device(void *stream, F f) {
+    return guarded([&] {
+        Workspace &w = workspace();
+        std::lock_guard<std::mutex> lk(w.mu);
+        hipStream_t s = stream ? (hipStream_t)stream : w.stream;
+        try {
+            f(w, s);
+        } catch (...) {
+            recover(w, s);
+            throw;
         }
-    }
+    });
 }
+
+// The CPU path after a device error: off unless the caller enables it (kbwt_set_cpu_fallback, or KBWT_CPU_FALLBACK=1
+// in the environment when the library first looks).
+static bool fallback_on() {
+    int v = g_cpu_fallback.load();
+    if (v < 0) {
+        const char *e = getenv("KBWT_CPU_FALLBACK");
+        v = e && e[0] == '1' ? 1 : 0;
+        int expect = -1;
+        if (!g_cpu_fallback.compare_exchange_strong(expect, v)) v = expect;
+    }
+    return v == 1;
+}
+
+// One block through the CPU path after its device work failed with rc (-2 or -3). Returns the block's new code.
+static int32_t cpu_fallback(bool fwd, const uint8_t *T, uint8_t *U, int32_t n, int32_t r, int32_t *Iout,
+                            const int32_t *Iin, int32_t threads, int32_t rc) {
+    if (rc == KBWT_OK || rc == KBWT_ERR_ARG || !fallback_on()) return rc;
+    const int32_t rc2 = fwd ? cpu_bwt_aux(T, U, n, r, Iout, threads) : cpu_unbwt_aux(T, U, n, r, Iin, threads);
+    if (rc2 != KBWT_OK) return rc;
+    if (g_counters.cpu_fallbacks++ == 0)
+        fprintf(stderr, "kbwt: a device call failed; its blocks run on the CPU path (KBWT_CPU_FALLBACK), counted by "
+                        "kbwt_get_counters\n");
+    return KBWT_OK;
+}
+
+// ---- concurrent single-block calls run as batches (kbwt_combine.h): the runner of one batch of one kind, from host
+// buffers packed back to back in the workspace's staging buffers
+static Combiner g_combiner;
+
 static void run_pending_batch(std::vector<Pending *> &b, bool fwd) {
-    if (b.empty()) return;
     const int32_t count = (int32_t)b.size();
     std::vector<kbwt_seg> segs(count);
     int64_t at = 0;
@@ -1285,19 +1352,17 @@ static void run_pending_batch(std::vector<Pending *> &b, bool fwd) {
     if (!fwd)
         for (int32_t k = 0, o = 0; k < count; o += (int32_t)nsamples(b[k]->n, b[k]->r), k++)
             memcpy(I.data() + o, b[k]->Iin, nsamples(b[k]->n, b[k]->r) * 4);
-    const int32_t rc = guarded([&] {
-        Workspace &w = workspace();
-        std::lock_guard<std::mutex> lk(w.mu);
+    const int32_t rc = on_device(nullptr, [&](Workspace &w, hipStream_t s) {
         grow(w.T, (size_t)at + 64);
         if (fwd) grow(w.U, (size_t)at + 64);
         std::vector<Span> sp(count);
         for (int32_t k = 0; k < count; k++) sp[k] = {const_cast<uint8_t *>(b[k]->T), (size_t)b[k]->n};
-        to_device(w, P<uint8_t>(w.T), sp, w.stream);
+        to_device(w, P<uint8_t>(w.T), sp, s);
         uint8_t *out = fwd ? P<uint8_t>(w.U) : P<uint8_t>(w.T);
-        if (fwd) forward_batch_dev(w, P<uint8_t>(w.T), out, segs.data(), count, I.data(), w.stream);
-        else inverse_batch_dev(w, P<uint8_t>(w.T), out, segs.data(), count, I.data(), w.stream);
+        if (fwd) forward_batch_dev(w, P<uint8_t>(w.T), out, segs.data(), count, I.data(), s);
+        else inverse_batch_dev(w, P<uint8_t>(w.T), out, segs.data(), count, I.data(), s);
         for (int32_t k = 0; k < count; k++) sp[k].p = b[k]->U;
-        to_host(w, sp, out, w.stream);
+        to_host(w, sp, out, s);
     });
     const std::string err = g_last_error;
     for (int32_t k = 0, o = 0; k < count; o += (int32_t)nsamples(b[k]->n, b[k]->r), k++) {
@@ -1307,68 +1372,55 @@ static void run_pending_batch(std::vector<Pending *> &b, bool fwd) {
     }
 }
 
-static int32_t combine(Pending &p) {
-    std::unique_lock<std::mutex> lk(g_comb_mu);
+// Never throws: a caller that runs a batch must always wake the others. A block that fails on its own (the combiner
+// runs the blocks of a failed batch again one by one) takes the CPU path when it is enabled.
+static void run_pending(std::vector<Pending *> &b, bool fwd) {
+    if (b.empty()) return;
     try {
-        g_comb_q.push_back(&p);
+        run_pending_batch(b, fwd);
     } catch (...) {
-        g_last_error = "host allocation failed";
-        return KBWT_ERR_NOMEM;
+        for (Pending *q : b) {
+            q->rc = KBWT_ERR_NOMEM;
+            q->err = "host allocation failed";
+        }
     }
-    while (!p.done) {
-        if (g_comb_busy) {
-            g_comb_cv.wait(lk);
-            continue;
+    if (b.size() == 1 && b[0]->rc != KBWT_OK) {
+        Pending *q = b[0];
+        if (cpu_fallback(fwd, q->T, q->U, q->n, q->r, q->I, q->Iin, q->threads, q->rc) == KBWT_OK) {
+            q->rc = KBWT_OK;
+            q->err.clear();
         }
-        // take the queue's head up to the batch limits (65535 blocks, packed length below 2^31); nothing below can
-        // fail once the lists are reserved, so the runner always wakes the others
-        std::vector<Pending *> fwd, inv, taken;
-        try {
-            fwd.reserve(g_comb_q.size());
-            inv.reserve(g_comb_q.size());
-            taken.reserve(g_comb_q.size());
-        } catch (...) {
-            g_comb_q.erase(std::remove(g_comb_q.begin(), g_comb_q.end(), &p), g_comb_q.end());
-            g_last_error = "host allocation failed";
-            return KBWT_ERR_NOMEM;
-        }
-        g_comb_busy = true;
-        int64_t tf = 0, ti = 0;
-        size_t k = 0;
-        for (; k < g_comb_q.size(); k++) {
-            Pending *q = g_comb_q[k];
-            int64_t &t = q->fwd ? tf : ti;
-            std::vector<Pending *> &v = q->fwd ? fwd : inv;
-            if (v.size() == 65535 || t + q->n > INT32_MAX - 64) break;
-            t += q->n;
-            v.push_back(q);
-            taken.push_back(q);
-        }
-        g_comb_q.erase(g_comb_q.begin(), g_comb_q.begin() + (long)k);
-        lk.unlock();
-        run_pending(fwd, true);
-        run_pending(inv, false);
-        lk.lock();
-        for (Pending *q : taken) q->done = true;
-        g_comb_busy = false;
-        g_comb_cv.notify_all();
     }
-    g_last_error = p.rc ? p.err : std::string();
-    return p.rc;
+}
+
+static int32_t combine(Pending &p) {
+    std::string err;
+    const int32_t rc = g_combiner.submit(p, run_pending, err);
+    g_last_error = err;
+    return rc;
 }
 
 extern "C" {
 
+// A block longer than one batch may hold (BATCH_MAX_TOTAL, 2^31 - 65 bytes) cannot be indexed by the device
+// kernels: the drop-in entry points report it as out of memory (-2) before anything is queued.
+static int32_t too_long(int32_t n) {
+    if ((int64_t)n <= BATCH_MAX_TOTAL) return 0;
+    g_last_error = "block longer than 2^31 - 65 bytes";
+    return KBWT_ERR_NOMEM;
+}
+
 int32_t kolmo_sais_bwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t r, int32_t *I,
                            int32_t threads) {
-    (void)threads;
     if (T == NULL || U == NULL || A == NULL || n < 0 || r < 2 || (r & (r - 1)) != 0 || I == NULL) return -1;
     if (n <= 1) {
         if (n == 1) U[0] = T[0];
         I[0] = n;
         return 0;
     }
+    if (too_long(n)) return KBWT_ERR_NOMEM;
     Pending p;
+    p.threads = threads;
     p.fwd = true;
     p.T = T;
     p.U = U;
@@ -1381,7 +1433,6 @@ int32_t kolmo_sais_bwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, 
 
 int32_t kolmo_sais_unbwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, int32_t r, const int32_t *I,
                              int32_t threads) {
-    (void)threads;
     if (T == NULL || U == NULL || A == NULL || n < 0 || ((r != n) && ((r < 2) || ((r & (r - 1)) != 0))) || I == NULL)
         return -1;
     if (n <= 1) {
@@ -1391,7 +1442,9 @@ int32_t kolmo_sais_unbwt_aux(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n
     }
     for (int64_t t = 0; t <= (n - 1) / r; ++t)
         if (I[t] <= 0 || I[t] > n) return -1;
+    if (too_long(n)) return KBWT_ERR_NOMEM;
     Pending p;
+    p.threads = threads;
     p.fwd = false;
     p.T = T;
     p.U = U;
@@ -1408,10 +1461,8 @@ int32_t kolmo_sais_unbwt(const uint8_t *T, uint8_t *U, int32_t *A, int32_t n, in
 
 int32_t kbwt_dev_bwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, int32_t *I, void *stream) {
     if (dT == NULL || dU == NULL || !forward_args_ok(n, r) || I == NULL) return -1;
-    return guarded([&] {
-        Workspace &w = workspace();
-        std::lock_guard<std::mutex> lk(w.mu);
-        hipStream_t s = stream ? (hipStream_t)stream : w.stream;
+    if (too_long(n)) return KBWT_ERR_NOMEM;
+    return on_device(stream, [&](Workspace &w, hipStream_t s) {
         kbwt_seg g{0, n, r};
         forward_batch_dev(w, dT, dU, &g, 1, I, s);
     });
@@ -1421,10 +1472,8 @@ int32_t kbwt_dev_unbwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r,
     if (dT == NULL || dU == NULL || !inverse_args_ok(n, r) || I == NULL) return -1;
     kbwt_seg g{0, n, r};
     if (!samples_ok(&g, 1, I)) return -1;
-    return guarded([&] {
-        Workspace &w = workspace();
-        std::lock_guard<std::mutex> lk(w.mu);
-        hipStream_t s = stream ? (hipStream_t)stream : w.stream;
+    if (too_long(n)) return KBWT_ERR_NOMEM;
+    return on_device(stream, [&](Workspace &w, hipStream_t s) {
         inverse_batch_dev(w, dT, dU, &g, 1, I, s);
     });
 }
@@ -1432,20 +1481,14 @@ int32_t kbwt_dev_unbwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r,
 int32_t kbwt_dev_bwt_aux_batch_features(const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count,
                                         int32_t *I, float *features, void *stream) {
     if (dT == NULL || dU == NULL || I == NULL || features == NULL || !batch_ok(segs, count, true)) return -1;
-    return guarded([&] {
-        Workspace &w = workspace();
-        std::lock_guard<std::mutex> lk(w.mu);
-        hipStream_t s = stream ? (hipStream_t)stream : w.stream;
+    return on_device(stream, [&](Workspace &w, hipStream_t s) {
         forward_batch_dev(w, dT, dU, segs, count, I, s, features);
     });
 }
 int32_t kbwt_dev_bwt_aux_batch(const uint8_t *dT, uint8_t *dU, const kbwt_seg *segs, int32_t count, int32_t *I,
                                void *stream) {
     if (dT == NULL || dU == NULL || I == NULL || !batch_ok(segs, count, true)) return -1;
-    return guarded([&] {
-        Workspace &w = workspace();
-        std::lock_guard<std::mutex> lk(w.mu);
-        hipStream_t s = stream ? (hipStream_t)stream : w.stream;
+    return on_device(stream, [&](Workspace &w, hipStream_t s) {
         forward_batch_dev(w, dT, dU, segs, count, I, s);
     });
 }
@@ -1454,10 +1497,7 @@ int32_t kbwt_dev_unbwt_aux_batch(const uint8_t *dT, uint8_t *dU, const kbwt_seg 
                                  const int32_t *I, void *stream) {
     if (dT == NULL || dU == NULL || I == NULL || !batch_ok(segs, count, false)) return -1;
     if (!samples_ok(segs, count, I)) return -1;
-    return guarded([&] {
-        Workspace &w = workspace();
-        std::lock_guard<std::mutex> lk(w.mu);
-        hipStream_t s = stream ? (hipStream_t)stream : w.stream;
+    return on_device(stream, [&](Workspace &w, hipStream_t s) {
         inverse_batch_dev(w, dT, dU, segs, count, I, s);
     });
 }
@@ -1502,21 +1542,43 @@ int32_t kbwt_autotune(const uint8_t *dT, const kbwt_seg *segs, int32_t count, vo
     for (int32_t b = 0; b < count; b++) total += segs[b].n;
     if (total < 2) return -1;
     const uint32_t cls = size_class((uint32_t)total);
-    return guarded([&] {
-        Workspace &w = workspace();
-        std::lock_guard<std::mutex> lk(w.mu);
-        hipStream_t s = stream ? (hipStream_t)stream : w.stream;
+    return on_device(stream, [&](Workspace &w, hipStream_t s) {
+        // the class's entry as it was, put back unless the search finishes (a trial that fails leaves nothing
+        // untested installed); the scratch outputs and events are released on every path
+        struct Restore {
+            uint32_t cls;
+            Tuning saved;
+            bool keep = false;
+            ~Restore() {
+                if (keep) return;
+                std::lock_guard<std::mutex> tl(g_tune_mu);
+                g_tune[cls] = saved;
+            }
+        } restore{cls, tuning_for_class(cls)};
+        struct DevBuf {
+            void *p = nullptr;
+            ~DevBuf() {
+                if (p) (void)hipFree(p);
+            }
+        } bU, bB;
+        struct Event {
+            hipEvent_t e = nullptr;
+            ~Event() {
+                if (e) (void)hipEventDestroy(e);
+            }
+        } ev0, ev1;
         const int64_t span = segs[count - 1].offset + segs[count - 1].n;
-        void *dU = nullptr, *dB = nullptr;
-        HIPCHK(hipMalloc(&dU, (size_t)span + 64));
-        HIPCHK(hipMalloc(&dB, (size_t)span + 64));
+        fault_point(FS_ALLOC);
+        HIPCHK(hipMalloc(&bU.p, (size_t)span + 64));
+        HIPCHK(hipMalloc(&bB.p, (size_t)span + 64));
+        void *dU = bU.p, *dB = bB.p;
         std::vector<int32_t> I;
         size_t ns = 0;
         for (int32_t b = 0; b < count; b++) ns += nsamples(segs[b].n, segs[b].r);
         I.resize(ns);
-        hipEvent_t e0, e1;
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventCreate(&ev0.e));
+        HIPCHK(hipEventCreate(&ev1.e));
+        hipEvent_t e0 = ev0.e, e1 = ev1.e;
         // one forward + inverse with tuning t installed for this size class; best of two timed runs after a warm-up
         auto trial = [&](const Tuning &t) {
             {
@@ -1573,11 +1635,8 @@ int32_t kbwt_autotune(const uint8_t *dT, const kbwt_seg *segs, int32_t count, vo
         {
             std::lock_guard<std::mutex> tl(g_tune_mu);
             g_tune[cls] = cur;
+            restore.keep = true;
         }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipFree(dU);
-        (void)hipFree(dB);
         best->sort_cfg = (int32_t)cur.sort_cfg;
         best->inv_dshift = (int32_t)cur.inv_dshift;
         best->seg_rank = (int32_t)cur.seg_rank;
@@ -1586,57 +1645,90 @@ int32_t kbwt_autotune(const uint8_t *dT, const kbwt_seg *segs, int32_t count, vo
     });
 }
 
-// Host-pointer batches: the span [first offset, last end) goes to the device once, keeping the offsets.
+// Host-pointer batches: the span [first offset, last end) goes to the device once, keeping the offsets. When the
+// device work fails and the CPU path is enabled, every block of the batch is computed there instead.
+struct DevSpan {   // a device buffer freed on every path out of a call
+    void *p = nullptr;
+    ~DevSpan() {
+        if (p) (void)hipFree(p);
+    }
+};
+
 int32_t kbwt_bwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs, int32_t count, int32_t *I) {
     if (T == NULL || U == NULL || I == NULL || !batch_ok(segs, count, true)) return -1;
     if (count == 0) return 0;
-    return guarded([&] {
-        Workspace &w = workspace();
-        std::lock_guard<std::mutex> lk(w.mu);
+    const int32_t rc = on_device(nullptr, [&](Workspace &w, hipStream_t s) {
         const int64_t lo = segs[0].offset, hi = segs[count - 1].offset + segs[count - 1].n;
-        Buf in, out;
+        DevSpan in, out;
+        fault_point(FS_ALLOC);
         HIPCHK(hipMalloc(&in.p, (size_t)(hi - lo) + 64));
         HIPCHK(hipMalloc(&out.p, (size_t)(hi - lo) + 64));
-        try {
-            to_device(w, P<uint8_t>(in), {{const_cast<uint8_t *>(T) + lo, (size_t)(hi - lo)}}, w.stream);
-            std::vector<kbwt_seg> g(segs, segs + count);
-            for (auto &x : g) x.offset -= lo;
-            forward_batch_dev(w, P<uint8_t>(in), P<uint8_t>(out), g.data(), count, I, w.stream);
-            to_host(w, {{U + lo, (size_t)(hi - lo)}}, P<uint8_t>(out), w.stream);
-            HIPCHK(hipStreamSynchronize(w.stream));
-        } catch (...) {
-            (void)hipFree(in.p);
-            (void)hipFree(out.p);
-            throw;
-        }
-        HIPCHK(hipFree(in.p));
-        HIPCHK(hipFree(out.p));
+        to_device(w, (uint8_t *)in.p, {{const_cast<uint8_t *>(T) + lo, (size_t)(hi - lo)}}, s);
+        std::vector<kbwt_seg> g(segs, segs + count);
+        for (auto &x : g) x.offset -= lo;
+        forward_batch_dev(w, (const uint8_t *)in.p, (uint8_t *)out.p, g.data(), count, I, s);
+        to_host(w, {{U + lo, (size_t)(hi - lo)}}, (const uint8_t *)out.p, s);
+        SYNCCHK(s);
     });
+    if (rc == KBWT_OK || !fallback_on()) return rc;
+    for (int32_t b = 0, k = 0; b < count; k += (int32_t)nsamples(segs[b].n, segs[b].r), b++)
+        if (cpu_fallback(true, T + segs[b].offset, U + segs[b].offset, segs[b].n, segs[b].r, I + k, nullptr, 1, rc) !=
+            KBWT_OK)
+            return rc;
+    g_last_error.clear();
+    return KBWT_OK;
 }
 
 int32_t kbwt_unbwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs, int32_t count, const int32_t *I) {
     if (T == NULL || U == NULL || I == NULL || !batch_ok(segs, count, false)) return -1;
     if (!samples_ok(segs, count, I)) return -1;
     if (count == 0) return 0;
-    return guarded([&] {
-        Workspace &w = workspace();
-        std::lock_guard<std::mutex> lk(w.mu);
+    const int32_t rc = on_device(nullptr, [&](Workspace &w, hipStream_t s) {
         const int64_t lo = segs[0].offset, hi = segs[count - 1].offset + segs[count - 1].n;
-        Buf io;
+        DevSpan io;
+        fault_point(FS_ALLOC);
         HIPCHK(hipMalloc(&io.p, (size_t)(hi - lo) + 64));
-        try {
-            to_device(w, P<uint8_t>(io), {{const_cast<uint8_t *>(T) + lo, (size_t)(hi - lo)}}, w.stream);
-            std::vector<kbwt_seg> g(segs, segs + count);
-            for (auto &x : g) x.offset -= lo;
-            inverse_batch_dev(w, P<uint8_t>(io), P<uint8_t>(io), g.data(), count, I, w.stream);
-            to_host(w, {{U + lo, (size_t)(hi - lo)}}, P<uint8_t>(io), w.stream);
-            HIPCHK(hipStreamSynchronize(w.stream));
-        } catch (...) {
-            (void)hipFree(io.p);
-            throw;
-        }
-        HIPCHK(hipFree(io.p));
+        to_device(w, (uint8_t *)io.p, {{const_cast<uint8_t *>(T) + lo, (size_t)(hi - lo)}}, s);
+        std::vector<kbwt_seg> g(segs, segs + count);
+        for (auto &x : g) x.offset -= lo;
+        inverse_batch_dev(w, (const uint8_t *)io.p, (uint8_t *)io.p, g.data(), count, I, s);
+        to_host(w, {{U + lo, (size_t)(hi - lo)}}, (const uint8_t *)io.p, s);
+        SYNCCHK(s);
     });
+    if (rc == KBWT_OK || !fallback_on()) return rc;
+    for (int32_t b = 0, k = 0; b < count; k += (int32_t)nsamples(segs[b].n, segs[b].r), b++)
+        if (cpu_fallback(false, T + segs[b].offset, U + segs[b].offset, segs[b].n, segs[b].r, nullptr, I + k, 1, rc) !=
+            KBWT_OK)
+            return rc;
+    g_last_error.clear();
+    return KBWT_OK;
+}
+
+int32_t kbwt_set_cpu_fallback(int32_t on) {
+    const bool was = fallback_on();
+    g_cpu_fallback.store(on ? 1 : 0);
+    return was ? 1 : 0;
+}
+
+int32_t kbwt_fault_inject(const char *spec) {
+    int site;
+    uint64_t at, count;
+    if (!parse_fault(spec, site, at, count)) return KBWT_ERR_ARG;
+    arm_fault(site, at, count);
+    return KBWT_OK;
+}
+
+int32_t kbwt_get_counters(kbwt_counters *out) {
+    if (!out) return KBWT_ERR_ARG;
+    uint64_t batches = 0, blocks = 0, retried = 0;
+    g_combiner.counts(batches, blocks, retried);
+    out->device_errors = g_counters.device_errors.load();
+    out->retried_blocks = (int64_t)retried;
+    out->cpu_fallbacks = g_counters.cpu_fallbacks.load();
+    out->injected_faults = g_counters.injected_faults.load();
+    out->combined_batches = (int64_t)batches;
+    out->combined_blocks = (int64_t)blocks;
+    return KBWT_OK;
 }
 
 void kbwt_profile(int32_t on) {
@@ -1675,7 +1767,7 @@ int32_t kbwt_dev_copy(void *dst, const void *src, size_t bytes, void *stream) {
         hipStream_t s = stream ? (hipStream_t)stream : w.stream;
         launch_copy16(dst, src, bytes, s);
         LAUNCHCHK();
-        HIPCHK(hipStreamSynchronize(s));
+        SYNCCHK(s);
     });
 }
 
