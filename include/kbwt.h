@@ -61,6 +61,15 @@ KBWT_API int32_t kolmo_sais_unbwt(const uint8_t *T, uint8_t *U, int32_t *A, int3
 KBWT_API int32_t kbwt_dev_bwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, int32_t *I, void *stream);
 KBWT_API int32_t kbwt_dev_unbwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, const int32_t *I, void *stream);
 
+/* kbwt_dev_unbwt_aux without any allocation. No counterpart in the reference: it is for callers that cannot allocate
+   device memory (code under HIP graph capture, tight decode loops, harnesses that forbid hipMalloc). It calls no
+   hipMalloc, hipFree or allocating library routine: every buffer of the call lives in a static device array of about
+   100 MiB that each device running the library holds until the process ends. n is at most 4194304 (4 MiB; larger
+   blocks return -1 with a message in kbwt_last_error). stream NULL is the null stream. Same semantics, codes and
+   output as kbwt_dev_unbwt_aux. Calls that overlap on one device share the array and race: serialise them. */
+KBWT_API int32_t kbwt_dev_unbwt_aux_noalloc(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, const int32_t *I,
+                                            void *stream);
+
 /* ---- batches (no counterpart in the reference: kolmo calls the single-block entry points once per block) ----
    count independent blocks in one launch sequence. Block b occupies bytes [offset, offset + n) of T and of U;
    blocks must not overlap and are given in increasing offset order. r: the block's sampling interval, as for the

@@ -99,6 +99,9 @@ def lib():
         L.kbwt_dev_unbwt_aux.restype = ctypes.c_int32
         L.kbwt_dev_unbwt_aux.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, _i32p,
                                          ctypes.c_void_p]
+        L.kbwt_dev_unbwt_aux_noalloc.restype = ctypes.c_int32
+        L.kbwt_dev_unbwt_aux_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                                 _i32p, ctypes.c_void_p]
         L.kbwt_last_stats.restype = ctypes.c_int32
         L.kbwt_last_stats.argtypes = [ctypes.POINTER(Stats)]
         L.kbwt_last_error.restype = ctypes.c_char_p
@@ -227,6 +230,18 @@ def dev_unbwt_aux(dU, r, I, dT=None, stream=None):
     I = np.ascontiguousarray(I, np.int32)
     s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
     _check(lib().kbwt_dev_unbwt_aux(dU.data_ptr(), dT.data_ptr(), n, r, _ptr(I, _i32p), s))
+    return dT
+
+
+def dev_unbwt_aux_noalloc(dU, r, I, dT=None, stream=None):
+    """kbwt_dev_unbwt_aux_noalloc: the device-resident inverse from the library's static scratch (n <= 4 MiB)."""
+    import torch
+    n = dU.numel()
+    if dT is None:
+        dT = torch.empty_like(dU)
+    I = np.ascontiguousarray(I, np.int32)
+    s = stream.cuda_stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    _check(lib().kbwt_dev_unbwt_aux_noalloc(dU.data_ptr(), dT.data_ptr(), n, r, _ptr(I, _i32p), s))
     return dT
 
 

@@ -962,6 +962,107 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
 
 // ------------------------------------------------------------------ inverse
 
+// The inverse's shapes for a batch: bl gets roff, spoff and nreg; one splitter row in D = 2^dshift; the first cap
+// bytes of each sublist are staged during the counting walk (the walk's time is its longest sublist, about
+// D * ln(splitters) steps, times the load latency), in a chunk-major stage of cap / 16 rows of 16-byte chunks, one per
+// splitter (sstride >= S of them), at most stage_max bytes in all.
+struct InvPlan {
+    uint32_t B, N, S, dshift, cap, iters, nht, nlt, nwt;
+    size_t sstride;
+    size_t o_ht, o_lt, o_wt, meta_bytes;   // metadata layout: blocks | hist tiles | lf tiles | walk tiles
+};
+InvPlan plan_inverse(BlockInfo *bl, uint32_t B, uint32_t N, uint32_t dshift, uint32_t cap, size_t stage_max) {
+    InvPlan p{};
+    p.B = B;
+    p.N = N;
+    p.dshift = dshift;
+    const uint32_t dmask = (1u << dshift) - 1u;
+    for (uint32_t b = 0; b < B; b++) {
+        bl[b].roff = bl[b].off + b;
+        bl[b].spoff = p.S;
+        bl[b].nreg = (bl[b].n >> dshift) + 1;
+        p.S += bl[b].nreg + ((bl[b].P & dmask) != 0 ? 1u : 0u);
+    }
+    p.sstride = ((size_t)p.S + 63) & ~(size_t)63;
+    if ((size_t)cap * p.sstride > stage_max) cap = (uint32_t)std::max<size_t>(16, stage_max / p.sstride);
+    p.cap = std::max(16u, cap & ~15u);           // the counting walk stages whole 16-step chunks
+    p.iters = rank_jump_iters(p.S);
+    p.o_ht = align16(B * sizeof(BlockInfo));
+    p.o_lt = p.o_ht + (B + 1) * 4;
+    p.o_wt = p.o_lt + (B + 1) * 4;
+    p.meta_bytes = p.o_wt + (B + 1) * 4;
+    return p;
+}
+// The metadata of the plan into host memory hm (meta_bytes); also sets the tile counts.
+void inverse_meta(InvPlan &p, const BlockInfo *bl, uint8_t *hm) {
+    const uint32_t B = p.B, dmask = (1u << p.dshift) - 1u;
+    memcpy(hm, bl, B * sizeof(BlockInfo));
+    p.nht = fill_tiles((uint32_t *)(hm + p.o_ht), B, [&](uint32_t b) { return (bl[b].n + HIST_TILE - 1) / HIST_TILE; });
+    p.nlt = fill_tiles((uint32_t *)(hm + p.o_lt), B, [&](uint32_t b) { return (bl[b].n + LF_TILE - 1) / LF_TILE; });
+    p.nwt = fill_tiles((uint32_t *)(hm + p.o_wt), B, [&](uint32_t b) {
+        return (bl[b].nreg + ((bl[b].P & dmask) != 0 ? 1u : 0u) + WALK_TILE - 1) / WALK_TILE;
+    });
+}
+
+// Device buffers of one inverse call. Lookback epochs: depoch (a device word a bump kernel moves on, for launch
+// sequences a graph replays) or, when depoch is null, the host-side epoch.
+struct InvBufs {
+    uint32_t *lf;            // N + 64 words
+    uint64_t *pair;          // N + 64
+    uint64_t *link[2];       // S + 1 each
+    uint32_t *len, *cont;    // S + 1 each
+    uint8_t *stage;          // cap * sstride
+    uint64_t *status;        // (N / LF_TILE + B + 1) * 256
+    uint32_t *counts;        // B * MAXP * 256 (cleared by the sequence)
+    uint32_t *cbase;         // B * 256
+    uint32_t *bad;           // B
+    uint16_t *blk;           // N (B > 1 only)
+    uint8_t *meta;           // meta_bytes
+    uint32_t *tile_counter;  // zero between launches
+    uint32_t *depoch;
+    uint32_t epoch;
+};
+size_t inv_status_words(uint32_t N, uint32_t B) { return ((size_t)std::max<uint32_t>(N, 64) + LF_TILE - 1) / LF_TILE + B; }
+
+// The launch sequence of one inverse (every kernel on stream s; wp: the workspace whose events time the launches
+// while profiling, or null).
+void inverse_launches(const InvPlan &p, const InvBufs &bf, const uint8_t *dL, uint8_t *dOut, Workspace *wp,
+                      hipStream_t s) {
+    const uint32_t B = p.B, N = p.N, S = p.S;
+    auto tm = [&](const char *name, uint64_t elems, auto fn) {
+        if (wp) timed(*wp, s, name, elems, fn);
+        else fn();
+    };
+    const BlockInfo *d_bl = (const BlockInfo *)bf.meta;
+    const Tiling thist{(const uint32_t *)(bf.meta + p.o_ht), d_bl, B};
+    const Tiling tlf{(const uint32_t *)(bf.meta + p.o_lt), d_bl, B};
+    const Tiling twalk{(const uint32_t *)(bf.meta + p.o_wt), d_bl, B};
+    if (bf.depoch) launch_epoch_bump(bf.depoch, s);
+    launch_fill_u8(reinterpret_cast<uint8_t *>(bf.counts), 0, (size_t)B * MAXP * 256 * 4, s);
+    if (B > 1) tm("fill_blk", N, [&] { launch_fill_blk(bf.blk, thist, p.nht, s); });
+    tm("byte_hist", N, [&] { launch_byte_hist(dL, thist, p.nht, bf.counts, s); });
+    tm("scan_digits", 256 * B, [&] { launch_scan_digits(bf.counts, B, 1, d_bl, 2, nullptr, bf.cbase, s); });
+    tm("lf_pass", N, [&] {
+        launch_lf_pass(dL, tlf, p.nlt, bf.cbase, bf.lf, bf.status, bf.tile_counter, bf.depoch ? 0u : bf.epoch,
+                       bf.depoch, s);
+    });
+    tm("lf_pairs", N, [&] { launch_lf_pairs(bf.lf, dL, tlf, p.nlt, p.dshift, bf.pair, s); });
+    tm("walk_count", N, [&] {
+        launch_walk_count(bf.pair, twalk, p.nwt, p.dshift, S, bf.cbase, bf.link[0], bf.len, bf.stage, p.sstride, p.cap,
+                          bf.cont, s);
+    });
+    int cur = 0;
+    for (uint32_t it = 0; it < p.iters; it++) {
+        tm("rank_jump", S, [&] { launch_rank_jump(bf.link[cur], bf.link[cur ^ 1], S, s); });
+        cur ^= 1;
+    }
+    launch_check_lists(bf.link[cur], d_bl, B, S, bf.bad, s);
+    tm("walk_emit", N, [&] {
+        launch_walk_emit(bf.lf, twalk, p.nwt, p.dshift, S, bf.link[cur], bf.len, bf.cbase, bf.stage, p.sstride, p.cap,
+                         bf.cont, bf.bad, dOut, s);
+    });
+}
+
 // bl: one BlockInfo per block with off, n (>= 2) and P (primary index) filled in; packed BWTs dL, output dOut (may
 // be dL).
 void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockInfo> &bl, uint32_t N, hipStream_t s) {
@@ -970,69 +1071,48 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     memset(&st, 0, sizeof st);   // the stats describe this call alone: the forward's fields stay zero
     st.n = (int32_t)N;
     st.blocks = (int32_t)B;
-    // one splitter row in D = 2^dshift; the first cap bytes of each sublist are staged during the counting walk
-    // (the walk's time is its longest sublist, about D * ln(splitters) steps, times the load latency)
     const uint32_t dshift = tuning_for(N).inv_dshift;
-    uint32_t cap = env_u32("KBWT_INV_CAP", 16u << dshift, 1, 1u << 16);
-    const uint32_t dmask = (1u << dshift) - 1u;
-    uint32_t S = 0;
-    for (uint32_t b = 0; b < B; b++) {
-        bl[b].roff = bl[b].off + b;
-        bl[b].spoff = S;
-        bl[b].nreg = (bl[b].n >> dshift) + 1;
-        S += bl[b].nreg + ((bl[b].P & dmask) != 0 ? 1u : 0u);
-    }
-    st.splitters = (int32_t)S;
-    const size_t NN = std::max<size_t>(N, 64);
+    InvPlan p = plan_inverse(bl.data(), B, N, dshift, env_u32("KBWT_INV_CAP", 16u << dshift, 1, 1u << 16), STAGE_MAX);
+    st.splitters = (int32_t)p.S;
+    const size_t NN = std::max<size_t>(N, 64), S = p.S;
     grow(w.V0, NN * 4);                        // lf
-    grow(w.pos[0], ((size_t)S + 1) * 4);       // sublist lengths
-    grow(w.pos[1], ((size_t)S + 1) * 4);       // continuation rows
-    grow(w.K0, ((size_t)S + 1) * 8);           // links (ping)
-    grow(w.K1, ((size_t)S + 1) * 8);           // links (pong)
-    // chunk-major stage: cap / 16 rows of 16-byte chunks, one per splitter (sstride >= S of them), at most STAGE_MAX
-    // bytes in all
-    const size_t sstride = ((size_t)S + 63) & ~(size_t)63;
-    if ((size_t)cap * sstride > STAGE_MAX) cap = (uint32_t)std::max<size_t>(16, STAGE_MAX / sstride);
-    cap = std::max(16u, cap & ~15u);           // the counting walk stages whole 16-step chunks
+    grow(w.pos[0], (S + 1) * 4);               // sublist lengths
+    grow(w.pos[1], (S + 1) * 4);               // continuation rows
+    grow(w.K0, (S + 1) * 8);                   // links (ping)
+    grow(w.K1, (S + 1) * 8);                   // links (pong)
     grow(w.pair, NN * 8);                      // two LF steps per row
-    grow(w.stage, (size_t)cap * sstride);
-    const size_t ntiles_max = (NN + LF_TILE - 1) / LF_TILE + B;
-    grow(w.status, ntiles_max * 256 * 8, true);
+    grow(w.stage, (size_t)p.cap * p.sstride);
+    grow(w.status, inv_status_words(N, B) * 256 * 8, true);
     grow(w.counts, (size_t)B * MAXP * 256 * 4, true);
     grow(w.cbase, (size_t)B * 256 * 4);
     grow(w.bad, (size_t)B * 4);
     if (B > 1) grow(w.blk, NN * 2);
-
-    // metadata layout: blocks | hist tiles | lf tiles | walk tiles; the bad flags come back behind it
-    const size_t o_ht = align16(B * sizeof(BlockInfo)), o_lt = o_ht + (B + 1) * 4, o_wt = o_lt + (B + 1) * 4;
-    const size_t bytes = o_wt + (B + 1) * 4;
-    uint8_t *hm = (uint8_t *)host_staging(w, bytes + (size_t)B * 4);
-    grow(w.meta, bytes);
-    uint8_t *dm = P<uint8_t>(w.meta);
-    uint32_t *h_bad = (uint32_t *)(hm + align16(bytes));
-    memcpy(hm, bl.data(), B * sizeof(BlockInfo));
-    const uint32_t nht = fill_tiles((uint32_t *)(hm + o_ht), B, [&](uint32_t b) { return (bl[b].n + HIST_TILE - 1) / HIST_TILE; });
-    const uint32_t nlt = fill_tiles((uint32_t *)(hm + o_lt), B, [&](uint32_t b) { return (bl[b].n + LF_TILE - 1) / LF_TILE; });
-    const uint32_t nwt = fill_tiles((uint32_t *)(hm + o_wt), B, [&](uint32_t b) {
-        return (bl[b].nreg + ((bl[b].P & dmask) != 0 ? 1u : 0u) + WALK_TILE - 1) / WALK_TILE;
-    });
-    HIPCHK(hipMemcpyAsync(dm, hm, bytes, hipMemcpyHostToDevice, s));
-    const BlockInfo *d_bl = (const BlockInfo *)dm;
-    const Tiling thist{(const uint32_t *)(dm + o_ht), d_bl, B};
-    const Tiling tlf{(const uint32_t *)(dm + o_lt), d_bl, B};
-    const Tiling twalk{(const uint32_t *)(dm + o_wt), d_bl, B};
-    const uint16_t *blk = B > 1 ? P<uint16_t>(w.blk) : nullptr;
+    uint8_t *hm = (uint8_t *)host_staging(w, align16(p.meta_bytes) + (size_t)B * 4);
+    grow(w.meta, p.meta_bytes);
+    uint32_t *h_bad = (uint32_t *)(hm + align16(p.meta_bytes));
+    inverse_meta(p, bl.data(), hm);
+    HIPCHK(hipMemcpyAsync(w.meta.p, hm, p.meta_bytes, hipMemcpyHostToDevice, s));
+    InvBufs bf{};
+    bf.lf = P<uint32_t>(w.V0);
+    bf.pair = P<uint64_t>(w.pair);
+    bf.link[0] = P<uint64_t>(w.K0);
+    bf.link[1] = P<uint64_t>(w.K1);
+    bf.len = P<uint32_t>(w.pos[0]);
+    bf.cont = P<uint32_t>(w.pos[1]);
+    bf.stage = P<uint8_t>(w.stage);
+    bf.status = P<uint64_t>(w.status);
+    bf.counts = P<uint32_t>(w.counts);
+    bf.cbase = P<uint32_t>(w.cbase);
+    bf.bad = P<uint32_t>(w.bad);
+    bf.blk = B > 1 ? P<uint16_t>(w.blk) : nullptr;
+    bf.meta = P<uint8_t>(w.meta);
+    bf.tile_counter = w.counters + 2;
+    bf.depoch = w.counters + 15;
 
     HIPCHK(hipEventRecord(w.ev0, s));
-    uint32_t *lf = P<uint32_t>(w.V0);
-    uint64_t *link[2] = {P<uint64_t>(w.K0), P<uint64_t>(w.K1)};
-    uint32_t *len = P<uint32_t>(w.pos[0]), *cont = P<uint32_t>(w.pos[1]);
-    uint8_t *stage = P<uint8_t>(w.stage);
-    const uint32_t iters = rank_jump_iters(S);
-    uint32_t *depoch = w.counters + 15;
     // the launch sequence depends only on these shapes and buffers (the data travel in the metadata buffer)
     const std::vector<uint64_t> key = {
-        B, N, S, dshift, cap, sstride, nht, nlt, nwt, (uintptr_t)dL, (uintptr_t)dOut, (uintptr_t)dm,
+        B, N, S, p.dshift, p.cap, p.sstride, p.nht, p.nlt, p.nwt, (uintptr_t)dL, (uintptr_t)dOut, (uintptr_t)w.meta.p,
         (uintptr_t)w.V0.p, (uintptr_t)w.K0.p, (uintptr_t)w.K1.p, (uintptr_t)w.pos[0].p, (uintptr_t)w.pos[1].p,
         (uintptr_t)w.stage.p, (uintptr_t)w.status.p, (uintptr_t)w.counts.p, (uintptr_t)w.cbase.p,
         (uintptr_t)w.bad.p, (uintptr_t)w.blk.p, (uintptr_t)w.counters, (uintptr_t)w.pair.p};
@@ -1043,37 +1123,15 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
         w.dev_epoch_calls = 0;
     }
     run_graph(w.inv_graph, key, s,
-              {"fill_blk", "byte_hist", "scan_digits", "lf_pass", "lf_pairs", "walk_count", "rank_jump", "walk_emit"}, [&] {
-        launch_epoch_bump(depoch, s);
-        launch_fill_u8(P<uint8_t>(w.counts), 0, (size_t)B * MAXP * 256 * 4, s);
-        if (B > 1) TIMED("fill_blk", N, launch_fill_blk(P<uint16_t>(w.blk), thist, nht, s));
-        TIMED("byte_hist", N, launch_byte_hist(dL, thist, nht, P<uint32_t>(w.counts), s));
-        TIMED("scan_digits", 256 * B,
-              launch_scan_digits(P<uint32_t>(w.counts), B, 1, d_bl, 2, nullptr, P<uint32_t>(w.cbase), s));
-        TIMED("lf_pass", N,
-              launch_lf_pass(dL, tlf, nlt, P<uint32_t>(w.cbase), lf, P<uint64_t>(w.status), w.counters + 2, 0u, depoch,
-                             s));
-        TIMED("lf_pairs", N, launch_lf_pairs(lf, dL, tlf, nlt, dshift, P<uint64_t>(w.pair), s));
-        TIMED("walk_count", N,
-              launch_walk_count(P<uint64_t>(w.pair), twalk, nwt, dshift, S, P<uint32_t>(w.cbase), link[0], len, stage,
-                                sstride, cap, cont, s));
-        int cur = 0;
-        for (uint32_t it = 0; it < iters; it++) {
-            TIMED("rank_jump", S, launch_rank_jump(link[cur], link[cur ^ 1], S, s));
-            cur ^= 1;
-        }
-        launch_check_lists(link[cur], d_bl, B, S, P<uint32_t>(w.bad), s);
-        TIMED("walk_emit", N,
-              launch_walk_emit(lf, twalk, nwt, dshift, S, link[cur], len, P<uint32_t>(w.cbase), stage, sstride, cap,
-                               cont, P<uint32_t>(w.bad), dOut, s));
-    });
+              {"fill_blk", "byte_hist", "scan_digits", "lf_pass", "lf_pairs", "walk_count", "rank_jump", "walk_emit"},
+              [&] { inverse_launches(p, bf, dL, dOut, &w, s); });
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(h_bad, w.bad.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
     SYNCCHK(s);
     bool anybad = false;
     for (uint32_t b = 0; b < B; b++) anybad |= h_bad[b] != 0;
     if (anybad) {   // strings that are no BWT decode to zeros: deterministic bytes that fail the caller's checksum
-        launch_zero_bad(dOut, blk, P<uint32_t>(w.bad), N, s);
+        launch_zero_bad(dOut, bf.blk, bf.bad, N, s);
         LAUNCHCHK();
     }
     HIPCHK(hipEventRecord(w.ev1, s));
@@ -1082,6 +1140,132 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     HIPCHK(hipEventElapsedTime(&ms, w.ev0, w.ev1));
     st.ms_inv = ms;
     prof_collect(w);
+}
+
+// ---- the inverse that allocates nothing: kbwt_dev_unbwt_aux_noalloc
+//
+// Every device buffer of one call comes from `scratch`, a file-scope device array of NOALLOC_BYTES (about 100 MiB),
+// sized for blocks of up to NOALLOC_MAX_N bytes with splitters at least every 8 rows and at most 8 bytes of staged
+// sublist heads per input byte; each device that runs the library holds it until the process ends. In host code the
+// array's name is only a stand-in: its device address comes from hipGetSymbolAddress once per device (with that
+// device current) and only that address is used. Metadata goes up from the caller's stack and the validity flag comes
+// back into it; the launch sequence is the inverse's own (inverse_launches) with lookback epochs counted on the host.
+// Calls that overlap on one device share the array and race.
+constexpr uint32_t NOALLOC_MAX_N = 1u << 22;
+constexpr uint32_t NOALLOC_MIN_DSHIFT = 3;
+constexpr size_t NOALLOC_SMAX = NOALLOC_MAX_N / 8 + 2;
+constexpr size_t NOALLOC_STAGE = (size_t)8 * NOALLOC_MAX_N;
+constexpr size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t NA_LF = 0;
+constexpr size_t NA_PAIR = NA_LF + al256(((size_t)NOALLOC_MAX_N + 64) * 4);
+constexpr size_t NA_LINK0 = NA_PAIR + al256(((size_t)NOALLOC_MAX_N + 64) * 8);
+constexpr size_t NA_LINK1 = NA_LINK0 + al256((NOALLOC_SMAX + 1) * 8);
+constexpr size_t NA_LEN = NA_LINK1 + al256((NOALLOC_SMAX + 1) * 8);
+constexpr size_t NA_CONT = NA_LEN + al256((NOALLOC_SMAX + 1) * 4);
+constexpr size_t NA_STAGE = NA_CONT + al256((NOALLOC_SMAX + 1) * 4);
+constexpr size_t NA_STATUS = NA_STAGE + al256(NOALLOC_STAGE);
+constexpr size_t NA_STATUS_BYTES = al256((((size_t)NOALLOC_MAX_N + LF_TILE - 1) / LF_TILE + 2) * 256 * 8);
+constexpr size_t NA_COUNTS = NA_STATUS + NA_STATUS_BYTES;
+constexpr size_t NA_CBASE = NA_COUNTS + al256((size_t)MAXP * 256 * 4);
+constexpr size_t NA_BAD = NA_CBASE + al256(256 * 4);
+constexpr size_t NA_META = NA_BAD + 256;
+constexpr size_t NA_COUNTERS = NA_META + 256;
+constexpr size_t NA_T = NA_COUNTERS + 256;     // an input or output that is not 16-byte aligned goes through here
+constexpr size_t NOALLOC_BYTES = NA_T + al256((size_t)NOALLOC_MAX_N + 64);
+
+static __device__ __align__(256) uint8_t scratch[NOALLOC_BYTES];
+
+struct NoAllocDevice {
+    std::mutex mu;
+    uint8_t *base = nullptr;   // the device address of scratch on this device
+    uint32_t epoch = 1;        // host-side lookback epochs of the calls on this device
+};
+NoAllocDevice g_noalloc[64];
+
+// The device address of scratch on the current device (found on the first call there, whose stream also clears the
+// lookback status words), and this call's lookback epoch (the status words are cleared again before epochs wrap).
+uint8_t *noalloc_scratch(hipStream_t s, uint32_t &epoch) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) fail(hipErrorInvalidDevice, "hipGetDevice");
+    NoAllocDevice &d = g_noalloc[dev];
+    std::lock_guard<std::mutex> lk(d.mu);
+    bool clear = false;
+    if (!d.base) {
+        void *p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        d.base = static_cast<uint8_t *>(p);
+        clear = true;
+    }
+    if (d.epoch >= EPOCH_HALF - 1) {
+        d.epoch = 1;
+        clear = true;
+    }
+    if (clear) {
+        launch_fill_u8(d.base + NA_STATUS, 0, NA_STATUS_BYTES, s);
+        LAUNCHCHK();
+    }
+    epoch = d.epoch++;
+    return d.base;
+}
+
+void inverse_noalloc(const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t primary, hipStream_t s) {
+    kbwt_stats &st = g_stats;
+    memset(&st, 0, sizeof st);
+    st.n = (int32_t)n;
+    st.blocks = 1;
+    if (n <= 1) {
+        if (n == 1 && dU != dT) HIPCHK(hipMemcpyAsync(dU, dT, 1, hipMemcpyDeviceToDevice, s));
+        SYNCCHK(s);
+        return;
+    }
+    uint32_t epoch = 0;
+    uint8_t *base = noalloc_scratch(s, epoch);
+    BlockInfo bl{};
+    bl.off = 0;
+    bl.n = n;
+    bl.P = primary;
+    const uint32_t dshift = std::max(NOALLOC_MIN_DSHIFT, tuning_for(n).inv_dshift);
+    InvPlan p = plan_inverse(&bl, 1, n, dshift, 16u << dshift, NOALLOC_STAGE);
+    st.splitters = (int32_t)p.S;
+    alignas(16) uint8_t hm[256];
+    static_assert(sizeof(BlockInfo) <= 64, "metadata of one block fits the stack buffer");
+    inverse_meta(p, &bl, hm);
+    InvBufs bf{};
+    bf.lf = reinterpret_cast<uint32_t *>(base + NA_LF);
+    bf.pair = reinterpret_cast<uint64_t *>(base + NA_PAIR);
+    bf.link[0] = reinterpret_cast<uint64_t *>(base + NA_LINK0);
+    bf.link[1] = reinterpret_cast<uint64_t *>(base + NA_LINK1);
+    bf.len = reinterpret_cast<uint32_t *>(base + NA_LEN);
+    bf.cont = reinterpret_cast<uint32_t *>(base + NA_CONT);
+    bf.stage = base + NA_STAGE;
+    bf.status = reinterpret_cast<uint64_t *>(base + NA_STATUS);
+    bf.counts = reinterpret_cast<uint32_t *>(base + NA_COUNTS);
+    bf.cbase = reinterpret_cast<uint32_t *>(base + NA_CBASE);
+    bf.bad = reinterpret_cast<uint32_t *>(base + NA_BAD);
+    bf.blk = nullptr;
+    bf.meta = base + NA_META;
+    bf.tile_counter = reinterpret_cast<uint32_t *>(base + NA_COUNTERS);
+    bf.depoch = nullptr;
+    bf.epoch = epoch;
+    // the walks read only the LF table, so the output may overwrite the input; the kernels take 16-byte aligned rows
+    const bool direct = ((uintptr_t)dT & 15) == 0 && (dT == dU || ((uintptr_t)dU & 15) == 0);
+    const uint8_t *src = direct ? dT : base + NA_T;
+    uint8_t *dst = direct ? dU : base + NA_T;
+    if (!direct) HIPCHK(hipMemcpyAsync(base + NA_T, dT, n, hipMemcpyDeviceToDevice, s));
+    launch_fill_u8(base + NA_COUNTERS, 0, 64, s);   // the tile counter starts at zero whatever a failed call left
+    HIPCHK(hipMemcpyAsync(bf.meta, hm, p.meta_bytes, hipMemcpyHostToDevice, s));
+    inverse_launches(p, bf, src, dst, nullptr, s);
+    LAUNCHCHK();
+    uint32_t h_bad = 0;
+    HIPCHK(hipMemcpyAsync(&h_bad, bf.bad, 4, hipMemcpyDeviceToHost, s));
+    SYNCCHK(s);
+    if (h_bad) {   // a string that is no BWT decodes to zeros, as in kbwt_dev_unbwt_aux
+        launch_zero_bad(dst, nullptr, bf.bad, n, s);
+        LAUNCHCHK();
+    }
+    if (!direct) HIPCHK(hipMemcpyAsync(dU, base + NA_T, n, hipMemcpyDeviceToDevice, s));
+    SYNCCHK(s);
 }
 
 // ------------------------------------------------------------------ batch plumbing
@@ -1475,6 +1659,27 @@ int32_t kbwt_dev_unbwt_aux(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r,
     if (too_long(n)) return KBWT_ERR_NOMEM;
     return on_device(stream, [&](Workspace &w, hipStream_t s) {
         inverse_batch_dev(w, dT, dU, &g, 1, I, s);
+    });
+}
+
+int32_t kbwt_dev_unbwt_aux_noalloc(const uint8_t *dT, uint8_t *dU, int32_t n, int32_t r, const int32_t *I,
+                                   void *stream) {
+    if (dT == NULL || dU == NULL || !inverse_args_ok(n, r) || I == NULL) return -1;
+    kbwt_seg g{0, n, r};
+    if (!samples_ok(&g, 1, I)) return -1;
+    if ((int64_t)n > (int64_t)NOALLOC_MAX_N) {
+        g_last_error = "kbwt_dev_unbwt_aux_noalloc: n exceeds its scratch limit of 4194304 bytes";
+        return KBWT_ERR_ARG;
+    }
+    return guarded([&] {
+        hipStream_t s = (hipStream_t)stream;
+        try {
+            inverse_noalloc(dT, dU, (uint32_t)n, (uint32_t)I[0], s);
+        } catch (...) {   // let what was queued finish before the error goes back
+            (void)hipStreamSynchronize(s);
+            (void)hipGetLastError();
+            throw;
+        }
     });
 }
 
