@@ -6,8 +6,11 @@
                    and size of the codec-18 stream (kolmo_bwt_compress with KOLMO_BWT_DEFAULTS).
   bsc_golden.json  the 63 fnv-1a hashes of libbsc 3.4.0 bsc_compress(LZP 15/72, BWT, static QLFC) streams over the
                    input kinds of tests/test_bwt.c:24-45, as the table holds them (data, not code).
+  batch_cfg4.json  the 512-block batch the benchmark times (seed 4, sizes log-uniform 4 KiB-16 MiB, Markov text,
+                   text + CSV, uniform bytes and DNA): per block its kind, n, r, SHA-256 of the block, and SHA-256 of
+                   its BWT bytes followed by its sampled indexes (int32 little-endian), from kolmo's own C.
 
-Run from the repository root after `make -C oracle`:  python tests/golden/make_fixtures.py
+Run from the repository root after `make -C oracle`:  python tests/golden/make_fixtures.py [--batch-only]
 """
 import json
 import os
@@ -55,6 +58,32 @@ def entry(name, gen, seed, n):
     }
 
 
+BATCH_CFG4 = {"seed": 4, "count": 512, "lo": 4096, "hi": 16 << 20, "kinds": ["markov2", "mixed", "random", "dna"]}
+
+
+def batch_fixture():
+    import hashlib
+    from concurrent.futures import ThreadPoolExecutor
+    b = BATCH_CFG4
+    blocks, kind_of, _ = K.generate_batch(b["seed"], b["count"], b["lo"], b["hi"], b["kinds"])
+    R = refs.ref()
+
+    def one(k):
+        T = blocks[k]
+        r = 1 << K.codec18_rlog(T.size)
+        rc, U, I = refs.sais_bwt(R, T, r, threads=2)
+        assert rc == 0, rc
+        h = hashlib.sha256(U.tobytes())
+        h.update(I.astype("<i4").tobytes())
+        return {"kind": {v: n for n, v in K.GEN_KINDS.items()}[int(kind_of[k])],
+                "n": int(T.size), "r": r, "sha_T": refs.sha256(T), "sha_UI": h.hexdigest()}
+
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        out = list(ex.map(one, range(len(blocks))))
+    return dict(b, source="oracle/_ref/libkolmo_ref.so (kolmo 0.0.2 sais.c, built from source)",
+                total=int(sum(x["n"] for x in out)), blocks=out)
+
+
 def bsc_golden():
     text = open(os.path.join(REF_TREE, "tests", "test_bwt_golden.h")).read()
     rows = re.findall(r"\{([^}]*)\}", text)
@@ -67,6 +96,10 @@ def bsc_golden():
 
 def main():
     assert refs.have_ref(), "build oracle/_ref first (make -C oracle)"
+    with open(os.path.join(HERE, "batch_cfg4.json"), "w") as f:
+        json.dump(batch_fixture(), f, indent=0)
+    if "--batch-only" in sys.argv:
+        return
     out = [entry(*x) for x in INPUTS]
     with open(os.path.join(HERE, "fixtures.json"), "w") as f:
         json.dump({"source": "oracle/_ref/libkolmo_ref.so (kolmo 0.0.2 sais.c + bwt_stream.c, built from source)",

@@ -45,3 +45,32 @@ def test_full_library_frames_match_published_digests(which):
         # every one of these frames has block-sorted blocks: the shim answered forward and inverse calls
         assert L.oracle_shim_calls(0) - before >= len(GOLD["frames"]) * 2
         assert L.oracle_shim_calls(1) > 0
+
+
+V12 = json.load(open(os.path.join(HERE, "golden", "v12_frames.json")))
+
+
+def v12_frame(f):
+    return open(os.path.join(HERE, "golden", "v12", f["file"]), "rb").read()
+
+
+def test_v12_frames_are_the_corpus_data():
+    assert len(V12["frames"]) == 34
+    for f in V12["frames"]:
+        assert hashlib.sha256(v12_frame(f)).hexdigest() == f["frame_sha256"], f["file"]
+
+
+@pytest.mark.parametrize("which", ["ref", "oracle"])
+def test_v12_frames_decode_to_the_originals(which):
+    """The frames of kolmo's frozen corpus (tests/v12) that hold block-sorted blocks decode to the originals' digests
+    through kolmo's full library, unchanged and with sais.c replaced by the oracle (whose inverse runs for each)."""
+    if not refs.have_full(which):
+        pytest.skip("oracle/_ref/libkolmo_full_*.so not built (kolmo tree absent)")
+    import numpy as np
+    L = refs.full(which)
+    for f in V12["frames"]:
+        before = L.oracle_shim_calls(1) if which == "oracle" else 0
+        back = refs.kolmo_decompress(L, np.frombuffer(v12_frame(f), np.uint8), f["n"], threads=2)
+        assert hashlib.sha256(back.tobytes()).hexdigest() == f["sha256"], f["file"]
+        if which == "oracle":
+            assert L.oracle_shim_calls(1) - before == f["inverse_calls"], f["file"]

@@ -121,3 +121,39 @@ def test_batch_workload_sample(gpu):
     rs = [1 << K.codec18_rlog(b.size) for b in blocks]
     say("64-block workload sample, total", sum(b.size for b in blocks))
     check_batch(blocks, rs, "workload sample")
+
+
+def test_gpu_batch_cfg4_full(gpu):
+    """The whole batch the benchmark times (512 blocks of 4 KiB-16 MiB, 946 MiB, seed 4) in one batched forward and
+    one batched inverse on device buffers: every block's BWT bytes and sampled indexes equal kolmo's own C
+    (tests/golden/batch_cfg4.json, written by oracle/_ref/libkolmo_ref.so), and every block comes back."""
+    import hashlib
+    import json
+    import os
+    import torch
+    fx = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "batch_cfg4.json")))
+    say("cfg4: generating", fx["count"], "blocks")
+    blocks, _, _ = K.generate_batch(fx["seed"], fx["count"], fx["lo"], fx["hi"], fx["kinds"])
+    sizes = [int(b.size) for b in blocks]
+    assert sizes == [e["n"] for e in fx["blocks"]] and sum(sizes) == fx["total"]
+    rs = [1 << K.codec18_rlog(n) for n in sizes]
+    assert rs == [e["r"] for e in fx["blocks"]]
+    segs, total = K.make_segs(sizes, rs)
+    dT = torch.from_numpy(np.concatenate(blocks)).cuda()
+    say("cfg4: batched forward of", total, "bytes")
+    dU, I = K.dev_bwt_aux_batch(dT, segs, len(blocks))
+    st = K.last_stats()
+    say("cfg4: rounds", st["rounds"], "active", st["active"][:8])
+    U = dU.cpu().numpy()
+    k = 0
+    for b, e in enumerate(fx["blocks"]):
+        ns = K.n_samples(sizes[b], rs[b])
+        o = segs[b].offset
+        h = hashlib.sha256(U[o:o + sizes[b]].tobytes())
+        h.update(np.asarray(I[k:k + ns], "<i4").tobytes())
+        assert h.hexdigest() == e["sha_UI"], f"block {b} ({e['kind']}, n {sizes[b]}) differs from kolmo's"
+        k += ns
+    say("cfg4: batched inverse")
+    back = K.dev_unbwt_aux_batch(dU, segs, len(blocks), I)
+    torch.cuda.synchronize()
+    assert torch.equal(back, dT), "batched inverse did not restore every block"

@@ -72,3 +72,20 @@ def test_gpu_container_multiblock_equals_reference(gpu, kind, seed, n, level):
     say("  device launches", {k: counts.get(k, 0) for k in ("bwt", "lf_pass")})
     # a frame of this kind and level sorts blocks of it: each forward ends with one k_bwt, each inverse builds LF once
     assert counts.get("bwt", 0) >= 1 and counts.get("lf_pass", 0) >= 1
+
+
+def test_gpu_v12_frames_decode_to_the_originals(gpu):
+    """kolmo's frozen-format corpus (tests/v12): every frame that holds block-sorted blocks (34 of the 357, written by
+    kolmo at its max, -8, -9 and ultra settings) decodes through the drop-in to the original's SHA-256 at one and four
+    threads, and each of its inverse BWTs ran on the device (the library's combined-call counter moves by the frame's
+    inverse calls)."""
+    V12 = json.load(open(os.path.join(HERE, "golden", "v12_frames.json")))
+    H = refs.full("hip")
+    for f in V12["frames"]:
+        frame = np.frombuffer(open(os.path.join(HERE, "golden", "v12", f["file"]), "rb").read(), np.uint8)
+        for threads in (1, 4):
+            say("v12", f["corpus_path"], "threads", threads)
+            c0 = K.counters()["combined_blocks"]
+            back = refs.kolmo_decompress(H, frame, f["n"], threads=threads)
+            assert hashlib.sha256(back.tobytes()).hexdigest() == f["sha256"], (f["file"], threads)
+            assert K.counters()["combined_blocks"] - c0 == f["inverse_calls"], f["file"]

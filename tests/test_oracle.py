@@ -198,3 +198,22 @@ def test_sampling_rule_matches_codec18_header():
         assert not flags & 4
         rlog, k = int(S[5]), int(S[6])
         assert rlog == K.codec18_rlog(n) and k == (n - 1) >> rlog
+
+
+def test_batch_cfg4_fixture_pinned():
+    """tests/golden/batch_cfg4.json (the benchmark's 512-block batch, written from kolmo's own C): the generator
+    reproduces every block, and the oracle gives the recorded BWT and indexes for a sample of them."""
+    import hashlib
+    import json
+    fx = json.load(open(os.path.join(HERE, "golden", "batch_cfg4.json")))
+    blocks, _, _ = K.generate_batch(fx["seed"], fx["count"], fx["lo"], fx["hi"], fx["kinds"])
+    assert len(blocks) == fx["count"] == len(fx["blocks"])
+    for T, e in zip(blocks, fx["blocks"]):
+        assert T.size == e["n"] and refs.sha256(T) == e["sha_T"]
+    small = sorted(range(len(blocks)), key=lambda b: blocks[b].size)
+    for b in small[:6] + small[200:203] + [small[-1]]:
+        e = fx["blocks"][b]
+        U, I = refs.oracle_bwt(blocks[b], e["r"])
+        h = hashlib.sha256(U.tobytes())
+        h.update(np.asarray(I, "<i4").tobytes())
+        assert h.hexdigest() == e["sha_UI"], (b, e["kind"], e["n"])
