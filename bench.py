@@ -236,6 +236,7 @@ def run_single(args, K):
     I = None
     for _ in range(max(1, args.warmup)):
         _, I = K.dev_bwt_aux(dT, r, dU=dU, stream=stream)
+        st = K.last_stats()                   # the forward's doubling profile (the inverse's call resets the stats)
         K.dev_unbwt_aux(dU, r, I, dT=dB, stream=stream)
     torch.cuda.synchronize()
     if not torch.equal(dB, dT):
@@ -260,7 +261,6 @@ def run_single(args, K):
     ms_step = elapsed / args.steps * 1e3
     value = n * args.steps / elapsed / 2**20
     roofline, path, dom, achieved = roofline_from(kernels, args.steps, ms_step, dom, drec, args.workload)
-    st = K.last_stats()
     mb = model_bytes(n, st)
     path["model_bytes_per_step"] = int(mb)
     path["model_frac"] = round(mb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
@@ -294,9 +294,9 @@ def run_batch(args, K):
     I = None
     for _ in range(max(1, args.warmup)):
         _, I = K.dev_bwt_aux_batch(dT, segs, count, dU=dU, stream=stream)
+        st = K.last_stats()                   # the batched forward's doubling profile
         K.dev_unbwt_aux_batch(dU, segs, count, I, dT=dB, stream=stream)
     torch.cuda.synchronize()
-    st = K.last_stats()                       # the batched forward's doubling profile
     if not torch.equal(dB, dT):
         raise SystemExit("batched inverse BWT did not restore the blocks")
     # the batched result of a few blocks against the single-block calls
