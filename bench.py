@@ -17,7 +17,8 @@ Workloads (sampling interval r by codec 18's rule for every block):
 Besides throughput it reports, for the dominant kernel, the algorithmic bytes per launch over its average duration
 measured with HIP events on its own stream inside the timed region (roofline), and kolmo's own CPU path
 (oracle/_ref, compiled from kolmo's sources) timed on this host's cores on a bounded sample of the same workload.
-The default cfg2 line also carries, under config: the batched form on cfg4 against one call per block ("batched"),
+The default cfg2 line also carries, under config: the 128 MiB deep-repeat block ("deep_repeats"), the batched form
+on cfg4 against one call per block ("batched"),
 the shipped router on the held-out router blocks ("router"), and the cfg2 block through the host-pointer entry
 points, PCIe included ("pcie_inclusive"), and the same block as 1 MiB blocks through those entry points from 16
 concurrent callers as kolmo's block workers make them, against one caller, from Python threads and from C threads
@@ -60,6 +61,12 @@ KERNEL_BYTES = {
     "emit_compact": 0.0,   # packs the next round's group list (a few bytes per group, not per row)
 }
 
+# The same with every random gather into a table far larger than an XCD's L2 charged one 128-byte line, the read
+# request size of the GPU's L2 (rocprofv3: TCC_EA0_RDREQ counts 128 bytes per request for these kernels): the counting
+# walk's pair loads (one per two steps), the doubling rounds' ISA[i + h] gathers. This line model is what measured HBM
+# traffic is checked against; the algorithmic one is what roofline.achieved counts.
+LINE_BYTES = dict(KERNEL_BYTES, walk_count=1.0 + 128.0 / 2, seg_gather=12.0 - 4 + 128, round_keys=32.0 - 4 + 128)
+
 WORKLOADS = {
     "cfg1": ("markov2", 1, 1 << 20, "one 1 MiB block (seed 1): order-2 Markov over 64 symbols, Dirichlet(0.3)"),
     "cfg2": ("mixed", 2, 32 << 20, "one 32 MiB block (seed 2): 70% order-3 Markov text (96 symbols, Dirichlet(0.2)) "
@@ -96,42 +103,116 @@ def traffic_from_profiles(kernel, workload):
     return None
 
 
-def cpu_baseline(blocks, rs, budget_s=20.0, reps=2):
-    """kolmo's own kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux (oracle/_ref, built from kolmo's C) on this host's cores,
-    over the blocks in order until about budget_s seconds of work (each block best of `reps`)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
+def cpu_model():
+    """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(n, 64))
+
+
+def cpu_time(R, blocks, rs, threads, budget_s, reps):
+    """kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux of kolmo's own C (oracle/_ref, built from kolmo's sources) over the
+    blocks in order, each best of `reps` after one warm-up, until about budget_s seconds of work; every round trip
+    checked. Returns (bytes, seconds, forward seconds, inverse seconds, blocks used)."""
     import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
     import refs
-    if not refs.have_ref():
-        return None
-    R = refs.ref()
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = max(1, min(threads, 64))
     done_bytes, done_s, fwd_s, inv_s, used = 0, 0.0, 0.0, 0.0, 0
     for T, r in zip(blocks, rs):
         best = None
-        for rep in range(reps):
+        for rep in range(reps + 1):
             t0 = time.perf_counter()
             rc, U, I = refs.sais_bwt(R, T, r, threads=threads)
             t1 = time.perf_counter()
             rc2, back = refs.sais_unbwt(R, U, r, I, threads=threads)
             t2 = time.perf_counter()
             assert rc == 0 and rc2 == 0 and np.array_equal(back, T)
-            if best is None or t2 - t0 < best[0]:
+            if rep > 0 and (best is None or t2 - t0 < best[0]):
                 best = (t2 - t0, t1 - t0, t2 - t1)
         done_bytes += T.size
         done_s += best[0]
         fwd_s += best[1]
         inv_s += best[2]
         used += 1
-        if done_s * reps >= budget_s:
+        if done_s * (reps + 1) >= budget_s:
             break
+    return done_bytes, done_s, fwd_s, inv_s, used
+
+
+def cpu_baseline(blocks, rs, budget_s=12.0, reps=3, cfg1=True):
+    """kolmo's own path on this host's cores, as the measurement plan asks: threads = nproc (the reported value) and
+    threads = 1, best of 3 after one warm-up, with the CPU model; beside it the 1 MiB cfg1 block at both thread counts
+    and kolmo's whole codec-18 stage (kolmo_bwt_compress / kolmo_bwt_decompress with its defaults) at nproc."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refs
+    if not refs.have_ref():
+        return None
+    R = refs.ref()
+    nproc = host_threads()
+    runs = {}
+    for th in (nproc, 1):
+        b, sec, f, i, used = cpu_time(R, blocks, rs, th, budget_s, reps)
+        runs[th] = {"value": round(b / sec / 2**20, 2), "fwd_ms": round(f * 1e3, 1), "inv_ms": round(i * 1e3, 1),
+                    "bytes": int(b), "blocks": used}
     what = (f"the full {blocks[0].size >> 20} MiB block" if len(blocks) == 1 else
-            f"the first {used} of {len(blocks)} blocks ({done_bytes / 2**20:.1f} MiB)")
-    return {"value": round(done_bytes / done_s / 2**20, 2), "unit": "MiB/s", "cores": threads, "kind": "reference",
-            "sample": f"{what}, kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux (kolmo 0.0.2 sais.c, gcc -O3 "
-                      f"-march=x86-64-v3) at threads={threads}, best of {reps}; fwd {fwd_s * 1e3:.0f} ms, "
-                      f"inv {inv_s * 1e3:.0f} ms"}
+            f"the first {runs[nproc]['blocks']} of {len(blocks)} blocks ({runs[nproc]['bytes'] / 2**20:.1f} MiB)")
+    out = {"value": runs[nproc]["value"], "unit": "MiB/s", "cores": nproc, "kind": "reference",
+           "sample": f"{what}, kolmo_sais_bwt_aux + kolmo_sais_unbwt_aux (kolmo 0.0.2 sais.c, gcc -O3 "
+                     f"-march=x86-64-v3) at threads={nproc}, best of {reps} after a warm-up; fwd "
+                     f"{runs[nproc]['fwd_ms']:.0f} ms, inv {runs[nproc]['inv_ms']:.0f} ms",
+           "cpu_model": cpu_model(), "threads_nproc": runs[nproc], "threads_1": runs[1]}
+    if cfg1:
+        import kolmo_amd as K
+        gen, seed, n, _ = WORKLOADS["cfg1"]
+        T1 = K.generate(gen, seed, n)
+        r1 = 1 << K.codec18_rlog(n)
+        c = {}
+        for th in (nproc, 1):
+            b, sec, f, i, _ = cpu_time(R, [T1], [r1], th, budget_s, reps)
+            c[f"threads_{'nproc' if th == nproc else 1}"] = {"value": round(b / sec / 2**20, 2),
+                                                             "fwd_ms": round(f * 1e3, 2), "inv_ms": round(i * 1e3, 2)}
+        out["cfg1"] = dict(c, workload="cfg1: " + WORKLOADS["cfg1"][3])
+    # the whole codec-18 stage (LZP, BWT, QLFC) for context, on the first block
+    T = blocks[0]
+    best_c = best_d = None
+    for _ in range(2):
+        t0 = time.perf_counter()
+        rc, S = refs.bwt18_compress(R, T, threads=nproc)
+        t1 = time.perf_counter()
+        rc2, back = refs.bwt18_decompress(R, S, T.size, threads=nproc)
+        t2 = time.perf_counter()
+        assert rc > 0 and rc2 == 0 and back.tobytes() == T.tobytes()
+        best_c = t1 - t0 if best_c is None else min(best_c, t1 - t0)
+        best_d = t2 - t1 if best_d is None else min(best_d, t2 - t1)
+    out["codec18_stage"] = {"compress": round(T.size / best_c / 2**20, 2), "decompress": round(T.size / best_d / 2**20, 2),
+                            "unit": "MiB/s", "threads": nproc,
+                            "what": "kolmo_bwt_compress / kolmo_bwt_decompress with KOLMO_BWT_DEFAULTS, best of 2"}
+    return out
+
+
+def traffic_total_from_profiles(workload):
+    """Measured HBM bytes per step of the whole path on `workload`, from the 'bytes per step, all kernels' line of
+    profiles/traffic_<workload>_*.md (rocprofv3 FETCH_SIZE and WRITE_SIZE, calibrated on a plain copy); None when no
+    such profile exists."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None
+    for f in sorted(os.listdir(pdir)):
+        if f.startswith(f"traffic_{workload}_") and f.endswith(".md"):
+            for line in open(os.path.join(pdir, f)):
+                m = re.match(r"Bytes per step, all kernels: ([0-9.eE+]+)", line)
+                if m:
+                    return float(m.group(1))
+    return None
 
 
 def roofline_from(kernels, steps, ms_step, dom, d, workload):
@@ -144,10 +225,18 @@ def roofline_from(kernels, steps, ms_step, dom, d, workload):
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom, workload), "kernel": dom,
                 "avg_launch_ms": round(avg_ms, 5), "algorithmic_bytes_per_launch": int(bytes_per_launch)}
     path_bytes = sum(KERNEL_BYTES.get(k, 0.0) * v["elems"] for k, v in kernels.items()) / steps
+    line_bytes = sum(LINE_BYTES.get(k, 0.0) * v["elems"] for k, v in kernels.items()) / steps
+    measured = traffic_total_from_profiles(workload)
     path = {"bytes_per_step": int(path_bytes), "achieved_GBs": round(path_bytes / (ms_step * 1e-3) / 1e9, 1),
             "frac": round(path_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "line_model_bytes_per_step": int(line_bytes),
+            "measured_traffic_per_step": measured,
+            "measured_over_line_model": round(measured / line_bytes, 3) if measured else None,
             "kernel_ms_per_step": {k: round(v["ms"] / steps, 4) for k, v in sorted(kernels.items(),
-                                   key=lambda kv: -kv[1]["ms"])}}
+                                   key=lambda kv: -kv[1]["ms"])},
+            "kernel_bytes_per_step": {k: int(KERNEL_BYTES.get(k, 0.0) * v["elems"] / steps) for k, v in kernels.items()},
+            "kernel_line_bytes_per_step": {k: int(LINE_BYTES.get(k, 0.0) * v["elems"] / steps)
+                                           for k, v in kernels.items()}}
     return roofline, path, dom, achieved
 
 
@@ -299,10 +388,12 @@ def run_batch(args, K):
     torch.cuda.synchronize()
     if not torch.equal(dB, dT):
         raise SystemExit("batched inverse BWT did not restore the blocks")
+    if args.calib_copy:
+        calib_copy(K, args.calib_copy, dev, stream)
     # the batched result of a few blocks against the single-block calls
     U = dU.cpu().numpy()
     k = 0
-    for b in range(count):
+    for b in range(0 if getattr(args, "no_per_block", False) else count):
         ns = K.n_samples(sizes[b], rs[b])
         if b < 8:
             u1, i1 = K.bwt_aux(blocks[b], rs[b])
@@ -329,28 +420,31 @@ def run_batch(args, K):
             _, Ib = K.dev_bwt_aux(t, rs[b], dU=u, stream=stream)
             K.dev_unbwt_aux(u, rs[b], Ib, dT=x, stream=stream)
 
-    log("bench: timing one call per block")
-    step_per_block()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    step_per_block()
-    torch.cuda.synchronize()
-    per_block_s = time.perf_counter() - t0
-    per_block = total / per_block_s / 2**20
+    per_block_s = per_block = None
+    if not getattr(args, "no_per_block", False):
+        log("bench: timing one call per block")
+        step_per_block()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step_per_block()
+        torch.cuda.synchronize()
+        per_block_s = time.perf_counter() - t0
+        per_block = total / per_block_s / 2**20
     roofline, path, dom, achieved = roofline_from(kernels, args.steps, ms_step, dom, drec, args.workload)
     mb = model_bytes(total, st)
     path["model_bytes_per_step"] = int(mb)
     path["model_frac"] = round(mb / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
-    log(f"bench: batched {value:.1f} MiB/s, one call per block {per_block:.1f} MiB/s ({value / per_block:.2f}x)")
+    if per_block:
+        log(f"bench: batched {value:.1f} MiB/s, one call per block {per_block:.1f} MiB/s ({value / per_block:.2f}x)")
     cpu = None
     if not args.no_cpu:
         log("bench: CPU baseline (kolmo's own C on this host)")
-        cpu = cpu_baseline(blocks, rs)
+        cpu = cpu_baseline(blocks, rs, cfg1=False)
     config = {"workload": f"{args.workload}: {desc}", "blocks": count, "total_bytes": total,
               "rounds": st["rounds"], "active": st["active"],
               "one_call_per_block": {"value": round(per_block, 2), "unit": "MiB/s",
-                                     "ms_per_step": round(per_block_s * 1e3, 3)},
-              "batched_over_per_block": round(value / per_block, 3)}
+                                     "ms_per_step": round(per_block_s * 1e3, 3)} if per_block else None,
+              "batched_over_per_block": round(value / per_block, 3) if per_block else None}
     return value, ms_step, config, roofline, path, cpu
 
 
@@ -471,7 +565,11 @@ def main():
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="also run one plain device copy of this many bytes (the reference for byte counters)")
     ap.add_argument("--no-extras", action="store_true",
-                    help="cfg2 only: skip the batched comparison, the router summary and the PCIe-inclusive rate")
+                    help="cfg2 only: skip the deep-repeat and batched lines, the router summary, the PCIe-inclusive "
+                         "rate and the concurrent calls")
+    ap.add_argument("--no-per-block", action="store_true",
+                    help="cfg4 only: skip the single-block calls (the one-call-per-block comparison and the check of "
+                         "8 blocks against them), for profiled runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): blocks are independent and a multi-GPU "
@@ -485,6 +583,14 @@ def main():
     if args.workload == "cfg2" and not args.no_extras:
         config["pcie_inclusive"] = pcie_inclusive(K)
         config["concurrent_calls"] = concurrent_calls(K)
+        dargs = argparse.Namespace(**vars(args))
+        dargs.workload, dargs.steps, dargs.warmup, dargs.no_cpu, dargs.calib_copy = "cfg3", 2, 1, True, 0
+        dv, dms, dcfg, droof, dpath, _ = run_single(dargs, K)
+        config["deep_repeats"] = {"workload": dcfg["workload"], "value": round(dv, 2), "unit": "MiB/s",
+                                  "ms_per_step": round(dms, 3), "rounds": dcfg["rounds"],
+                                  "bit_exact_vs_reference": dcfg["bit_exact_vs_reference"],
+                                  "dominant_kernel": droof["kernel"], "dominant_frac": droof["frac"],
+                                  "model_frac": dpath["model_frac"]}
         bargs = argparse.Namespace(**vars(args))
         bargs.workload, bargs.steps, bargs.warmup, bargs.no_cpu = "cfg4", 2, 1, True
         bv, bms, bcfg, _, _, _ = run_batch(bargs, K)
