@@ -526,33 +526,80 @@ def concurrent_calls(K, workers=16, reps=3):
                     "from Python threads, and from C threads (c_threads)"}
 
 
-def router_summary():
+def router_summary(sample=256):
     """The shipped router on the held-out fifth of the router workload (8192 blocks of 256 KiB-4 MiB, seed 5): its
-    bytes against exhaustive best-per-block and its coder throughput, from the committed labels and GPU features."""
+    bytes against exhaustive best-per-block (every class coded, the smallest kept) and its coder throughput. The
+    features of `sample` held-out blocks are recomputed live here (regenerated blocks, one batched forward with
+    features, timed on the device stream) and must equal the committed ones; their device seconds per byte, scaled to
+    the held-out bytes, are the features' share of the router's time. Coder times are the committed single-thread CPU
+    times of kolmo's coders. Also the same figures on the out-of-distribution set (real files of kolmo's tests, from
+    its committed features: rebuilding those files needs kolmo's own library)."""
     import numpy as np
+    import torch
+    import kolmo_amd as K
     from kolmo_amd import router as R
     lab_p = os.path.join(ROOT, "tests", "golden", "router_labels.npz")
     feat_p = os.path.join(ROOT, "tests", "golden", "router_features.npz")
+    ood_p = os.path.join(ROOT, "tests", "golden", "router_ood.npz")
     if not (os.path.exists(lab_p) and os.path.exists(feat_p) and os.path.exists(R.MODEL)):
         return None
     lab = np.load(lab_p)
-    fz = np.load(feat_p)
-    F = fz["features"]
+    F = np.load(feat_p)["features"]
     P, th = R.load_model()
-    sizes = R.outcome_sizes(lab["n"], lab["size_bwt"], lab["size_klzma"])
+    protos, radius = R.load_gate()
+    sizes = R.outcome_sizes(lab["n"], lab["size_bwt"], lab["size_klzma"], lab["size_bwt2"])
     test = np.arange(len(F)) % 5 == 4
-    tb, tk = lab["t_bwt"].astype(np.float64), lab["t_klzma"].astype(np.float64)
+    tb, tk, t2 = (lab[k].astype(np.float64) for k in ("t_bwt", "t_klzma", "t_bwt2"))
+    # live features of a sample of held-out blocks
+    idx = np.flatnonzero(test)[:sample]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=16) as ex:
+        blocks = list(ex.map(lambda i: R.workload_block(lab["kind"][i], lab["seed"][i], lab["n"][i], model_seed=5),
+                             idx))
+    sizes_s = [int(b.size) for b in blocks]
+    segs, total = K.make_segs(sizes_s, [1 << K.codec18_rlog(x) for x in sizes_s])
+    dT = torch.from_numpy(np.concatenate(blocks + [np.zeros(1, np.uint8)])).cuda()
+    stream = torch.cuda.current_stream()
+    R.dev_features_batch(dT, segs, len(blocks), stream=stream)        # warm-up (workspace, code objects)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    _, _, Fl = R.dev_features_batch(dT, segs, len(blocks), stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_live = e0.elapsed_time(e1) * 1e-3
+    if np.max(np.abs(Fl - F[idx])) > 1e-6:
+        raise SystemExit("live router features differ from the committed ones")
+    share = t_live / total * float(lab["n"][test].sum())
     t0 = time.perf_counter()
-    dec = R.decide(R.predict(F[test], P), th)
+    dec = R.decide(R.predict(F[test], P), th, None if protos is None else R.gate(F[test], protos), radius)
     t_pred = time.perf_counter() - t0
-    share = float(fz["gpu_seconds"]) * lab["n"][test].sum() / lab["n"].sum()
-    ev = R.evaluate(dec, sizes[test], tb[test], tk[test], t_features=[share + t_pred])
-    ex = R.evaluate(np.full(test.sum(), -1), sizes[test], tb[test], tk[test])
-    log(f"bench: router {ev['loss_pct']:+.3f}% bytes at {ex['seconds'] / ev['seconds']:.2f}x exhaustive throughput")
-    return {"blocks": int(test.sum()), "size_loss_pct": round(ev["loss_pct"], 4),
-            "throughput_over_exhaustive": round(ex["seconds"] / ev["seconds"], 3),
-            "fallback_blocks": ev["fallback_blocks"],
-            "what": "held-out fifth of 8192 blocks; coder times are single-thread CPU times of kolmo's coders"}
+    ev = R.evaluate(dec, sizes[test], tb[test], tk[test], t2[test], t_features=[share + t_pred])
+    exh = R.evaluate(np.full(test.sum(), -1), sizes[test], tb[test], tk[test], t2[test])
+    log(f"bench: router {ev['loss_pct']:+.3f}% bytes at {exh['seconds'] / ev['seconds']:.2f}x exhaustive throughput "
+        f"(features {total / t_live / 2**20:.0f} MiB/s live)")
+    out = {"blocks": int(test.sum()), "size_loss_pct": round(ev["loss_pct"], 4),
+           "throughput_over_exhaustive": round(exh["seconds"] / ev["seconds"], 3),
+           "fallback_blocks": ev["fallback_blocks"], "classes": "block sorting, KLZMA, store, block sorting of halves",
+           "gate": None if protos is None else {"prototypes": int(len(protos)), "radius": round(radius, 4)},
+           "features_live": {"blocks": len(blocks), "bytes": int(total), "device_seconds": round(t_live, 5),
+                             "MiB_per_s": round(total / t_live / 2**20, 1)},
+           "what": "held-out fifth of 8192 blocks; coder times are single-thread CPU times of kolmo's coders; the "
+                   "features' share is timed live on a sample"}
+    if os.path.exists(ood_p):
+        d = np.load(ood_p)
+        if "features" in d:
+            so = R.outcome_sizes(d["n"], d["size_bwt"], d["size_klzma"], d["size_bwt2"])
+            to = [d[k].astype(np.float64) for k in ("t_bwt", "t_klzma", "t_bwt2")]
+            go = None if protos is None else R.gate(d["features"], protos)
+            eo = R.evaluate(R.decide(R.predict(d["features"], P), th, go, radius), so, *to)
+            xo = R.evaluate(np.full(len(so), -1), so, *to)
+            bo = R.evaluate(np.full(len(so), R.BWT), so, *to)
+            out["out_of_distribution"] = {"blocks": int(len(so)), "size_loss_pct": round(eo["loss_pct"], 4),
+                                          "always_bwt_loss_pct": round(bo["loss_pct"], 4),
+                                          "throughput_over_exhaustive": round(xo["seconds"] / eo["seconds"], 3),
+                                          "fallback_blocks": eo["fallback_blocks"]}
+    return out
 
 
 def main():

@@ -90,18 +90,19 @@ KBWT_API int32_t kbwt_dev_unbwt_aux_batch(const uint8_t *dT, uint8_t *dU, const 
                                           const int32_t *I, void *stream);
 
 /* ---- learned router (no counterpart in the reference) ----
-   Picks the coder of a level-8 block, the choice src/core/blocks.c:390-411 and :473-507 make today by coding a slice
-   of the block both ways: class 0 block sorting (codec 18, kolmo_bwt_compress), 1 KLZMA (codec 24,
-   kolmo_klzma_compress level 8), 2 store the block. The choice is recorded per block, so coders, containers and the
-   decoder are unchanged; exhaustive best-per-block (code both, keep the smaller) stays the fallback for blocks the
-   network is unsure of.
+   Picks the coder and the block size of a level-8 block, the choice src/core/blocks.c:390-411 and :473-507 make today
+   by coding a slice of the block both ways: class 0 block sorting of the whole block (codec 18, kolmo_bwt_compress),
+   1 KLZMA (codec 24, kolmo_klzma_compress level 8), 2 store the block, 3 block sorting at half the block size (its two
+   halves as two codec-18 blocks). The choice is recorded per block, so coders, containers and the decoder are
+   unchanged; exhaustive best-per-block (code every way, keep the smallest) stays the fallback for blocks the network
+   is unsure of.
    Features, KBWT_ROUTER_FEATURES fp32 per block: sqrt(byte count / n) for the 256 byte values, the order-1 entropy
    H(T[i] | T[i-1]) / 8, and m_r / n for r = 1..5 (the share of suffixes still in rank groups at the start of
-   doubling round r). Network: 262 -> 64 -> 64 -> 3, ReLU, softmax; KBWT_ROUTER_PARAMS fp32 parameters, input-major
-   (W1[k*64+j], b1, W2[k*64+j], b2, W3[k*3+c], b3). All router arrays below are host memory. */
+   doubling round r). Network: 262 -> 64 -> 64 -> 4, ReLU, softmax; KBWT_ROUTER_PARAMS fp32 parameters, input-major
+   (W1[k*64+j], b1, W2[k*64+j], b2, W3[k*4+c], b3). All router arrays below are host memory. */
 #define KBWT_ROUTER_FEATURES 262
-#define KBWT_ROUTER_CLASSES 3
-#define KBWT_ROUTER_PARAMS 21187
+#define KBWT_ROUTER_CLASSES 4
+#define KBWT_ROUTER_PARAMS 21252
 #define KBWT_ROUTER_GRAD_MAX 4096   /* samples of one kbwt_router_grad call and of one training batch */
 
 /* kbwt_dev_bwt_aux_batch, and the router's features of every block into features[count * KBWT_ROUTER_FEATURES]. */
@@ -121,8 +122,21 @@ KBWT_API int32_t kbwt_router_train(const float *X, const int32_t *y, const float
 /* Gradient of that mean loss at params over ns <= KBWT_ROUTER_GRAD_MAX samples, and the loss. */
 KBWT_API int32_t kbwt_router_grad(const float *X, const int32_t *y, const float *w, int32_t ns, const float *params,
                                   float *grad, float *loss);
-/* Class probabilities, probs[ns * 3]. */
+/* Class probabilities, probs[ns * KBWT_ROUTER_CLASSES]. */
 KBWT_API int32_t kbwt_router_predict(const float *X, int32_t ns, const float *params, float *probs);
+/* The novelty gate: dist[s] = Euclidean distance from the features of sample s to the nearest of np prototype feature
+   vectors (protos[np * KBWT_ROUTER_FEATURES]). A block far from every prototype is unlike the blocks the network was
+   trained on, and its decision falls back to exhaustive best-per-block. */
+KBWT_API int32_t kbwt_router_gate(const float *X, int32_t ns, const float *protos, int32_t np, float *dist);
+/* The router at kolmo's decision site (src/core/blocks.c:390-411, 473-507), for a frame's blocks at once: features
+   of each host block (blocks[b], n[b] bytes; one batched forward on the device), the network, the gate, and
+   decision[b] = the class of highest probability, or -1 when that probability is below threshold or the block lies
+   farther than radius from every prototype (protos NULL: no gate); on -1 the caller codes the block every way and
+   keeps the smallest (exhaustive best-per-block). probs (NULL or count * KBWT_ROUTER_CLASSES floats) gets the
+   probabilities. params, threshold, protos and radius: a trained model (kolmo_amd/data/router_model.npz). */
+KBWT_API int32_t kbwt_router_route(const uint8_t *const *blocks, const int32_t *n, int32_t count, const float *params,
+                                   float threshold, const float *protos, int32_t nprotos, float radius,
+                                   int32_t *decision, float *probs);
 
 /* ---- autotuner (no counterpart in the reference) ----
    Launch parameters per size class (ceil(log2) of a call's total length): sort_cfg, the radix pass's workgroup and

@@ -141,10 +141,14 @@ __global__ __launch_bounds__(64) void k_mlp(const float *__restrict__ X, const i
     }
     __syncthreads();
     if (j == 0) {
-        const float m = fmaxf(pz[0], fmaxf(pz[1], pz[2]));
-        const float e0 = __expf(pz[0] - m), e1 = __expf(pz[1] - m), e2 = __expf(pz[2] - m);
-        const float se = e0 + e1 + e2;
-        const float p[NC] = {e0 / se, e1 / se, e2 / se};
+        float m = pz[0];
+        for (int c = 1; c < NC; c++) m = fmaxf(m, pz[c]);
+        float p[NC], se = 0.f;
+        for (int c = 0; c < NC; c++) {
+            p[c] = __expf(pz[c] - m);
+            se += p[c];
+        }
+        for (int c = 0; c < NC; c++) p[c] /= se;
         const float w = wt ? wt[row] : 1.f;
         const int lab = y ? y[row] : 0;
         if (probs)
@@ -193,6 +197,38 @@ __global__ __launch_bounds__(256) void k_reduce_adam(const float *__restrict__ G
         V[p] = v;
         P[p] -= lr * (m / bc1) / (sqrtf(v / bc2) + eps);
     }
+}
+
+// The novelty gate: dist[s] = the Euclidean distance from sample s to the nearest of np prototype feature vectors
+// (protos[p * NF ..]). One wave per sample; lane l sums the squared differences of features l, l + 64, ...; the wave
+// adds them up with shuffles. The byte-histogram features are square roots of byte shares, so over them the distance
+// is sqrt(2) times the Hellinger distance of the byte distributions.
+__global__ __launch_bounds__(64) void k_gate(const float *__restrict__ X, uint32_t ns, const float *__restrict__ protos,
+                                             uint32_t np, float *__restrict__ dist) {
+    const uint32_t s = blockIdx.x;
+    if (s >= ns) return;
+    const int lane = threadIdx.x;
+    constexpr int PER = (NF + 63) / 64;
+    float x[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) x[q] = lane + 64 * q < NF ? X[(size_t)s * NF + lane + 64 * q] : 0.f;
+    float best = 3.4e38f;
+    for (uint32_t p = 0; p < np; p++) {
+        float acc = 0.f;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const float d = lane + 64 * q < NF ? x[q] - protos[(size_t)p * NF + lane + 64 * q] : 0.f;
+            acc = fmaf(d, d, acc);
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        best = fminf(best, acc);
+    }
+    if (lane == 0) dist[s] = sqrtf(best);
+}
+
+void launch_gate(const float *X, uint32_t ns, const float *protos, uint32_t np, float *dist, hipStream_t s) {
+    if (ns) k_gate<<<ns, 64, 0, s>>>(X, ns, protos, np, dist);
 }
 
 void launch_mlp(bool grad, const float *X, const int32_t *y, const float *wt, const uint32_t *idx, uint32_t ns,

@@ -127,6 +127,68 @@ int32_t kbwt_router_grad(const float *X, const int32_t *y, const float *w, int32
     });
 }
 
+int32_t kbwt_router_gate(const float *X, int32_t ns, const float *protos, int32_t np, float *dist) {
+    if (X == NULL || protos == NULL || dist == NULL || ns < 0 || np <= 0) return KBWT_ERR_ARG;
+    if (ns == 0) return 0;
+    return guarded_router([&] {
+        DBuf dX((size_t)ns * ROUTER_NF * 4), dP((size_t)np * ROUTER_NF * 4), dD((size_t)ns * 4);
+        RCHK(hipMemcpy(dX.p, X, (size_t)ns * ROUTER_NF * 4, hipMemcpyHostToDevice));
+        RCHK(hipMemcpy(dP.p, protos, (size_t)np * ROUTER_NF * 4, hipMemcpyHostToDevice));
+        launch_gate(dX.as<float>(), (uint32_t)ns, dP.as<float>(), (uint32_t)np, dD.as<float>(), nullptr);
+        RCHK(hipGetLastError());
+        RCHK(hipMemcpy(dist, dD.p, (size_t)ns * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int32_t kbwt_router_route(const uint8_t *const *blocks, const int32_t *n, int32_t count, const float *params,
+                          float threshold, const float *protos, int32_t nprotos, float radius, int32_t *decision,
+                          float *probs) {
+    if (blocks == NULL || n == NULL || params == NULL || decision == NULL || count < 0 || count > 65535 ||
+        (protos != NULL && nprotos <= 0))
+        return KBWT_ERR_ARG;
+    int64_t total = 0;
+    for (int32_t b = 0; b < count; b++) {
+        if (n[b] < 0 || (n[b] > 0 && blocks[b] == NULL)) return KBWT_ERR_ARG;
+        total += n[b];
+    }
+    if (total >= ((int64_t)1 << 31) - 64) return KBWT_ERR_ARG;
+    if (count == 0) return 0;
+    return guarded_router([&] {
+        // the frame's blocks back to back on the device, one sample each (r: the power of two at or above n)
+        std::vector<kbwt_seg> segs(count);
+        int64_t at = 0;
+        for (int32_t b = 0; b < count; b++) {
+            int32_t r = 2;
+            while (r < n[b] && r < (1 << 30)) r <<= 1;
+            segs[b] = {at, n[b], r};
+            at += n[b];
+        }
+        DBuf dT((size_t)at + 64), dU((size_t)at + 64);
+        for (int32_t b = 0; b < count; b++)
+            if (n[b]) RCHK(hipMemcpy(dT.as<uint8_t>() + segs[b].offset, blocks[b], (size_t)n[b], hipMemcpyHostToDevice));
+        std::vector<int32_t> I((size_t)count * 2);
+        std::vector<float> F((size_t)count * ROUTER_NF), P((size_t)count * ROUTER_NC);
+        int32_t rc = kbwt_dev_bwt_aux_batch_features(dT.as<uint8_t>(), dU.as<uint8_t>(), segs.data(), count, I.data(),
+                                                     F.data(), nullptr);
+        if (rc != 0) throw RouterError{rc};
+        rc = kbwt_router_predict(F.data(), count, params, P.data());
+        if (rc != 0) throw RouterError{rc};
+        std::vector<float> D(protos ? count : 0);
+        if (protos) {
+            rc = kbwt_router_gate(F.data(), count, protos, nprotos, D.data());
+            if (rc != 0) throw RouterError{rc};
+        }
+        for (int32_t b = 0; b < count; b++) {
+            const float *p = P.data() + (size_t)b * ROUTER_NC;
+            int32_t best = 0;
+            for (int32_t c = 1; c < ROUTER_NC; c++)
+                if (p[c] > p[best]) best = c;
+            decision[b] = p[best] >= threshold && (!protos || D[b] <= radius) ? best : -1;
+        }
+        if (probs) memcpy(probs, P.data(), P.size() * 4);
+    });
+}
+
 int32_t kbwt_router_train(const float *X, const int32_t *y, const float *w, int32_t ns,
                           const kbwt_router_train_cfg *cfg, float *params, float *epoch_loss) {
     if (X == NULL || y == NULL || cfg == NULL || params == NULL || ns <= 0 || cfg->epochs < 0 || cfg->batch <= 0 ||

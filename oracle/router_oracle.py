@@ -3,8 +3,8 @@
 Only tests/, __graft_entry__.smoke() and bench.py's CPU leg may import this module. The router has no counterpart in
 kolmo; what it restates is textbook mathematics, in float64 numpy:
 
-  - the network: 262 -> 64 -> 64 -> 3 with ReLU and softmax, parameters in the library's input-major layout
-    (W1[k*64+j], b1, W2[k*64+j], b2, W3[k*3+c], b3), and the gradient of the weighted mean cross-entropy by
+  - the network: 262 -> 64 -> 64 -> 4 with ReLU and softmax, parameters in the library's input-major layout
+    (W1[k*64+j], b1, W2[k*64+j], b2, W3[k*4+c], b3), and the gradient of the weighted mean cross-entropy by
     back-propagation. Pinned by central finite differences (tests/test_router_oracle.py) and by the closed form of the
     softmax cross-entropy gradient (p - onehot) at the logits;
   - the features: sqrt(byte count / n), the order-1 entropy H(T[i] | T[i-1]) / 8 from bigram counts (pinned by
@@ -19,7 +19,7 @@ kolmo; what it restates is textbook mathematics, in float64 numpy:
 """
 import numpy as np
 
-NF, NH, NC = 262, 64, 3
+NF, NH, NC = 262, 64, 4
 NPARAM = NF * NH + NH + NH * NH + NH + NH * NC + NC
 ROUNDS = 5
 

@@ -17,11 +17,23 @@ the middle, offset ((n - 512 KiB) / 2) & ~63) with block sorting and with KLZMA 
 src/codecs/native.c:38) and keeps block sorting only when it is at least 2% smaller. The second pass (--slices)
 records those slice sizes and times and the text flag.
 
+The block-size choice: block sorting on the block's two halves instead of the whole block (each half its own
+codec-18 stream; the halves split at n/2). The third pass (--halves) records the two halves' summed size and time.
+
 Output: router_labels.npz (no pickles): kind, seed, n, size_bwt, size_klzma (-1: stored), t_bwt, t_klzma (seconds);
-after --slices also text (0/1), slice_bwt, slice_klzma1 (-1: stored or not coded), t_slice_bwt, t_slice_klzma1.
+after --slices also text (0/1), slice_bwt, slice_klzma1 (-1: stored or not coded), t_slice_bwt, t_slice_klzma1;
+after --halves also size_bwt2 (-1: a half could not be shrunk), t_bwt2.
 
     python tests/golden/make_router_labels.py [--count 8192] [--procs 8]
     python tests/golden/make_router_labels.py --slices [--procs 8]
+    python tests/golden/make_router_labels.py --halves [--procs 8]
+
+Heterogeneous blocks (--pairs N, seed 6, sizes log-uniform 256 KiB-4 MiB): each block is two of the six generators'
+blocks of half its size one after the other (kolmo_amd.router.workload_block), the case where coding the halves apart
+can pay. They get every field above in one pass and go to router_pairs.npz; they add to the router's training data
+and have their own held-out fifth, while the 8192 generated blocks keep theirs.
+
+    python tests/golden/make_router_labels.py --pairs 1024 [--procs 8]
 """
 import argparse
 import ctypes
@@ -36,6 +48,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.dirname(HERE))
+from kolmo_amd import router as R  # noqa: E402
 
 ROUTER_SEED, ROUTER_LO, ROUTER_HI = 5, 256 << 10, 4 << 20
 ROUTER_KINDS = ["markov2", "csvcols", "logs", "bf16", "random", "dna"]
@@ -56,7 +69,7 @@ def _label(args):
     kind, seed, n = args
     import kolmo_amd as K
     import refs
-    T = K.generate(int(kind), int(seed), int(n), model_seed=ROUTER_SEED)
+    T = R.workload_block(kind, seed, n, model_seed=ROUTER_SEED)
     t0 = time.perf_counter()
     rc, _ = refs.bwt18_compress(_lib, T, threads=1)
     t1 = time.perf_counter()
@@ -85,7 +98,7 @@ def _slice(args):
     kind, seed, n = args
     import kolmo_amd as K
     import refs
-    T = K.generate(int(kind), int(seed), int(n), model_seed=ROUTER_SEED)
+    T = R.workload_block(kind, seed, n, model_seed=ROUTER_SEED)
     text = looks_text_latin(T)
     if text or n < (64 << 10):
         return text, -1, -1, 0.0, 0.0
@@ -99,6 +112,73 @@ def _slice(args):
     k = _lib.kolmo_klzma_compress(S.ctypes.data, S.size, out.ctypes.data, out.size, 1)
     t2 = time.perf_counter()
     return text, (rc if rc > 0 else -1), (int(k) if k > 0 else -1), t1 - t0, t2 - t1
+
+
+def bwt_halves(T):
+    """Both halves of T through kolmo_bwt_compress (KOLMO_BWT_DEFAULTS): summed size (-1 when a half is not
+    compressible) and seconds."""
+    import refs
+    h = T.size // 2
+    t0 = time.perf_counter()
+    r1, _ = refs.bwt18_compress(_lib, np.ascontiguousarray(T[:h]), threads=1)
+    r2, _ = refs.bwt18_compress(_lib, np.ascontiguousarray(T[h:]), threads=1)
+    t1 = time.perf_counter()
+    return (r1 + r2 if r1 > 0 and r2 > 0 else -1), t1 - t0
+
+
+def _halves(args):
+    kind, seed, n = args
+    return bwt_halves(R.workload_block(kind, seed, n, model_seed=ROUTER_SEED))
+
+
+def main_halves(a):
+    d = dict(np.load(a.out))
+    jobs = list(zip(d["kind"].tolist(), d["seed"].tolist(), d["n"].tolist()))
+    res = []
+    t0 = time.time()
+    with Pool(a.procs, initializer=_init) as pool:
+        for i, r in enumerate(pool.imap(_halves, jobs, chunksize=4)):
+            res.append(r)
+            if (i + 1) % 1024 == 0:
+                print(f"{i + 1}/{len(jobs)} blocks, {time.time() - t0:.0f} s", flush=True)
+    res = np.array(res, dtype=np.float64)
+    d.update(size_bwt2=res[:, 0].astype(np.int64), t_bwt2=res[:, 1].astype(np.float32))
+    np.savez_compressed(a.out, **d)
+    print("wrote", a.out, f"{time.time() - t0:.0f} s")
+
+
+PAIRS_SEED = 6
+
+
+def _all(args):
+    return _label(args) + _slice(args) + _halves(args)
+
+
+def main_pairs(a):
+    import kolmo_amd as K
+    sizes, ka, seeds = K.batch_plan(PAIRS_SEED, a.pairs, ROUTER_LO, ROUTER_HI, ROUTER_KINDS)
+    _, kb, _ = K.batch_plan(PAIRS_SEED + 1, a.pairs, ROUTER_LO, ROUTER_HI, ROUTER_KINDS)
+    ids = [K.GEN_KINDS[k] for k in ROUTER_KINDS]
+    kb = np.where(kb == ka, np.array([ids[(ids.index(int(x)) + 1) % len(ids)] for x in ka]), kb)
+    kinds = (R.PAIR_BASE + 16 * ka + kb).astype(np.int32)
+    jobs = list(zip(kinds.tolist(), seeds.tolist(), sizes.tolist()))
+    res = []
+    t0 = time.time()
+    with Pool(a.procs, initializer=_init) as pool:
+        for i, r in enumerate(pool.imap(_all, jobs, chunksize=2)):
+            res.append(r)
+            if (i + 1) % 128 == 0:
+                print(f"{i + 1}/{len(jobs)} blocks, {time.time() - t0:.0f} s", flush=True)
+    r = np.array(res, dtype=np.float64)
+    out = os.path.join(HERE, "router_pairs.npz")
+    np.savez_compressed(out, kind=kinds, seed=seeds.astype(np.uint64), n=sizes.astype(np.int64),
+                        size_bwt=r[:, 0].astype(np.int64), size_klzma=r[:, 1].astype(np.int64),
+                        t_bwt=r[:, 2].astype(np.float32), t_klzma=r[:, 3].astype(np.float32),
+                        text=r[:, 4].astype(np.int8), slice_bwt=r[:, 5].astype(np.int64),
+                        slice_klzma1=r[:, 6].astype(np.int64), t_slice_bwt=r[:, 7].astype(np.float32),
+                        t_slice_klzma1=r[:, 8].astype(np.float32), size_bwt2=r[:, 9].astype(np.int64),
+                        t_bwt2=r[:, 10].astype(np.float32))
+    print("wrote", out, f"{time.time() - t0:.0f} s")
 
 
 def main_slices(a):
@@ -125,9 +205,15 @@ def main():
     ap.add_argument("--procs", type=int, default=8)
     ap.add_argument("--out", default=os.path.join(HERE, "router_labels.npz"))
     ap.add_argument("--slices", action="store_true")
+    ap.add_argument("--halves", action="store_true")
+    ap.add_argument("--pairs", type=int, default=0)
     a = ap.parse_args()
+    if a.pairs:
+        return main_pairs(a)
     if a.slices:
         return main_slices(a)
+    if a.halves:
+        return main_halves(a)
     import kolmo_amd as K
     sizes, kinds, seeds = K.batch_plan(ROUTER_SEED, a.count, ROUTER_LO, ROUTER_HI, ROUTER_KINDS)
     jobs = list(zip(kinds.tolist(), seeds.tolist(), sizes.tolist()))

@@ -24,7 +24,7 @@ def small_problem(seed, ns=7):
     rng = np.random.default_rng(seed)
     P = rng.normal(0, 0.3, RO.NPARAM)
     X = rng.random((ns, RO.NF))
-    y = rng.integers(0, 3, ns)
+    y = rng.integers(0, RO.NC, ns)
     w = rng.random(ns) + 0.5
     return P, X, y, w
 
@@ -49,10 +49,10 @@ def test_gradient_matches_finite_differences():
 def test_output_bias_gradient_closed_form():
     P, X, y, w = small_problem(3, ns=11)
     p, _ = RO.forward(P, X)
-    onehot = np.eye(3)[y]
+    onehot = np.eye(RO.NC)[y]
     expect = np.mean(w[:, None] * (p - onehot), axis=0)
     _, g = RO.loss_grad(P, X, y, w)
-    assert np.allclose(g[-3:], expect, atol=1e-14)
+    assert np.allclose(g[-RO.NC:], expect, atol=1e-14)
 
 
 def test_order1_entropy_closed_forms():
@@ -121,15 +121,42 @@ def test_doubling_order_on_plain_blocks():
 
 
 def test_labels_weights_decisions():
-    n = np.array([1000, 1000, 1000, 1000])
-    sizes = R.outcome_sizes(n, [500, 700, -1, 300], [600, 650, -1, 300])
-    assert sizes.tolist() == [[500, 600, 1000], [700, 650, 1000], [1000, 1000, 1000], [300, 300, 1000]]
-    assert R.labels(sizes).tolist() == [0, 1, 2, 0]
+    n = np.array([1000, 1000, 1000, 1000, 1000])
+    sizes = R.outcome_sizes(n, [500, 700, -1, 300, 400], [600, 650, -1, 300, 500], [520, 720, -1, 300, 380])
+    assert sizes.tolist() == [[500, 600, 1000, 520], [700, 650, 1000, 720], [1000, 1000, 1000, 1000],
+                              [300, 300, 1000, 300], [400, 500, 1000, 380]]
+    # ties go to storing, then the whole block's sorting, then KLZMA, then the halves
+    assert R.labels(sizes).tolist() == [0, 1, 2, 0, 3]
     w = R.weights(sizes)
-    assert abs(w.mean() - 1) < 1e-6 and w[0] > w[1]   # 0: choosing wrong costs 100 or 500 bytes, 1: 50 or 350
-    probs = np.array([[0.9, 0.05, 0.05], [0.4, 0.35, 0.25], [0.2, 0.2, 0.6], [0.1, 0.8, 0.1]])
+    assert abs(w.mean() - 1) < 1e-6 and w[0] > w[1]   # 0: choosing wrong costs 20-500 bytes, 1: 50-350
+    probs = np.array([[0.9, 0.05, 0.03, 0.02], [0.4, 0.35, 0.2, 0.05], [0.2, 0.1, 0.6, 0.1], [0.1, 0.8, 0.05, 0.05],
+                      [0.1, 0.1, 0.1, 0.7]])
     d = R.decide(probs, 0.5)
-    assert d.tolist() == [0, -1, 2, 1]
-    ev = R.evaluate(d, sizes, np.array([1.0, 1, 1, 1]), np.array([4.0, 4, 4, 4]))
-    assert ev["bytes"] == 500 + 650 + 1000 + 300 and ev["best_bytes"] == 500 + 650 + 1000 + 300
-    assert ev["seconds"] == 1 + 5 + 0 + 4 and ev["fallback_blocks"] == 1
+    assert d.tolist() == [0, -1, 2, 1, 3]
+    one, four, two = np.ones(5), np.full(5, 4.0), np.full(5, 2.0)
+    ev = R.evaluate(d, sizes, one, four, two)
+    assert ev["bytes"] == 500 + 650 + 1000 + 300 + 380 and ev["best_bytes"] == ev["bytes"]
+    assert ev["seconds"] == 1 + (1 + 4 + 2) + 0 + 4 + 2 and ev["fallback_blocks"] == 1
+    # the gate: a block farther than the radius from every prototype is coded every way, whatever its probability
+    g = R.decide(probs, 0.5, dist=np.array([0.1, 0.1, 0.3, 0.2, 0.25]), radius=0.2)
+    assert g.tolist() == [0, -1, -1, 1, -1]
+
+
+def test_prototypes_farthest_point_sampling():
+    """Farthest-point sampling on points of a line: from 0 it picks the far end, then the middle, then the quarter
+    points; the coverage radius is the largest distance from a point to its nearest pick."""
+    X = np.zeros((9, 3))
+    X[:, 0] = np.arange(9)
+    pick, cover = R.prototypes(X, 2)
+    assert pick.tolist() == [0, 8] and cover == 4.0
+    pick, cover = R.prototypes(X, 3)
+    assert pick.tolist() == [0, 8, 4] and cover == 2.0
+    pick, cover = R.prototypes(X, 5)
+    assert sorted(pick.tolist()) == [0, 2, 4, 6, 8] and cover == 1.0
+    pick, cover = R.prototypes(X, 50)
+    assert sorted(pick.tolist()) == list(range(9)) and cover == 0.0
+    # two clusters: two prototypes land one in each, and the coverage is a cluster's own spread
+    rng = np.random.default_rng(3)
+    Y = np.concatenate([rng.normal(0, 0.01, (50, 4)), rng.normal(5, 0.01, (50, 4))])
+    pick, cover = R.prototypes(Y, 2)
+    assert {int(p) // 50 for p in pick} == {0, 1} and cover < 0.1
