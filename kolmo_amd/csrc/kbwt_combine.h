@@ -1,8 +1,12 @@
 // Gathers concurrent single-block calls into batches. kolmo's block workers call kolmo_sais_bwt_aux and
 // kolmo_sais_unbwt_aux from several threads at once (src/core/compress.c:231-256, src/core/decode.c:404-410). A call
-// queues its block; a caller that finds no batch running takes every queued block up to the batch limits, runs the
-// forward blocks as one batch and the inverse blocks as another, and wakes the others. Each block's result is the
-// single-block result, so batching changes nothing but the launch count.
+// queues its block; a caller that finds a free lane and no batch still staging its inputs takes every queued block up
+// to the batch limits, runs the forward blocks as one batch and the inverse blocks as another on that lane, and wakes
+// the others. Each block's result is the single-block result, so batching changes nothing but the launch count.
+//
+// Up to `lanes` batches are in flight at once (two by default), so that one batch's host work overlaps another's
+// device work. The runner tells the combiner through Intake::done() when its batch's inputs are staged; only then may
+// the next batch form, so the blocks that queue meanwhile go into it together instead of into batches of one.
 //
 // Failures stay per block: when a batch of several blocks fails, each of its blocks runs again on its own, so one
 // block's failure is not every caller's, and a batch that fails as a whole does not fail blocks that succeed alone.
@@ -21,6 +25,7 @@ namespace kbwt {
 
 constexpr int32_t BATCH_MAX_BLOCKS = 65535;
 constexpr int64_t BATCH_MAX_TOTAL = (int64_t)INT32_MAX - 64;   // packed length of one batch (so of one block too)
+constexpr int COMBINE_LANES = 2;
 
 struct Pending {
     bool fwd = true;
@@ -35,10 +40,32 @@ struct Pending {
     std::string err;
 };
 
+class Combiner;
+
+// Handed to the runner with its batch: done() says that the batch's inputs are staged, so the next batch may form.
+// Calls after the first do nothing; staged() tells whether it has come (the blocks a failed batch runs again one by
+// one stage after it).
+class Intake {
+public:
+    explicit Intake(Combiner *c) : c_(c) {}
+    inline void done();
+    bool staged() const { return over_; }
+
+private:
+    Combiner *c_;
+    bool over_ = false;
+};
+
 class Combiner {
 public:
-    // Runner: void run(std::vector<Pending *> &batch, bool fwd) runs one batch of one kind and sets every block's rc
-    // and err; it must not throw. Returns p's rc; err gets p's message (empty when rc == 0).
+    explicit Combiner(int lanes = COMBINE_LANES) : lanes_(lanes < 1 ? 1 : lanes > 8 ? 8 : lanes) {
+        free_ = (1u << lanes_) - 1u;
+    }
+
+    // Runner: void run(std::vector<Pending *> &batch, bool fwd, int lane, Intake &in) runs one batch of one kind on
+    // lane `lane` (0 <= lane < lanes; no other batch holds it meanwhile), calls in.done() once the batch's inputs are
+    // staged (optional: the combiner calls it after the runner returns), and sets every block's rc and err; it must
+    // not throw. Returns p's rc; err gets p's message (empty when rc == 0).
     template <class Runner> int32_t submit(Pending &p, Runner &run, std::string &err) {
         std::unique_lock<std::mutex> lk(mu_);
         try {
@@ -48,7 +75,7 @@ public:
             return -2;
         }
         while (!p.done) {
-            if (busy_) {
+            if (intake_ || free_ == 0 || q_.empty()) {
                 cv_.wait(lk);
                 continue;
             }
@@ -62,7 +89,6 @@ public:
                 err = "host allocation failed";
                 return -2;
             }
-            busy_ = true;
             int64_t tf = 0, ti = 0;
             size_t k = 0;
             for (; k < q_.size(); k++) {
@@ -80,19 +106,24 @@ public:
                 q->rc = -2;
                 q->err = "block longer than one batch may hold";
                 q->done = true;
-                busy_ = false;
                 cv_.notify_all();
                 continue;
             }
             q_.erase(q_.begin(), q_.begin() + (long)k);
             batches_++;
             blocks_ += taken.size();
+            int lane = 0;
+            while (!(free_ & (1u << lane))) lane++;
+            free_ &= ~(1u << lane);
+            intake_ = true;
             lk.unlock();
-            const size_t re = run_isolated(fwd, true, run) + run_isolated(inv, false, run);
+            Intake in(this);
+            const size_t re = run_isolated(fwd, true, run, lane, in) + run_isolated(inv, false, run, lane, in);
+            in.done();
             lk.lock();
             retried_ += re;
             for (Pending *q : taken) q->done = true;
-            busy_ = false;
+            free_ |= 1u << lane;
             cv_.notify_all();
         }
         try {
@@ -111,10 +142,18 @@ public:
     }
 
 private:
+    friend class Intake;
+    void intake_over() {
+        std::lock_guard<std::mutex> lk(mu_);
+        intake_ = false;
+        cv_.notify_all();
+    }
+
     // Returns the blocks run again on their own.
-    template <class Runner> static size_t run_isolated(std::vector<Pending *> &b, bool fwd, Runner &run) {
+    template <class Runner>
+    static size_t run_isolated(std::vector<Pending *> &b, bool fwd, Runner &run, int lane, Intake &in) {
         if (b.empty()) return 0;
-        run(b, fwd);
+        run(b, fwd, lane, in);
         if (b.size() < 2) return 0;
         bool failed = false;
         for (Pending *q : b) failed |= q->rc != 0;
@@ -124,7 +163,7 @@ private:
             q->err.clear();
             try {
                 std::vector<Pending *> one(1, q);
-                run(one, fwd);
+                run(one, fwd, lane, in);
             } catch (...) {
                 q->rc = -2;
                 q->err = "host allocation failed";
@@ -136,8 +175,16 @@ private:
     std::mutex mu_;
     std::condition_variable cv_;
     std::vector<Pending *> q_;
-    bool busy_ = false;
+    int lanes_;
+    uint32_t free_;            // lanes no batch holds
+    bool intake_ = false;      // a batch is forming or staging its inputs
     uint64_t batches_ = 0, blocks_ = 0, retried_ = 0;
 };
+
+inline void Intake::done() {
+    if (over_) return;
+    over_ = true;
+    c_->intake_over();
+}
 
 }  // namespace kbwt

@@ -132,6 +132,49 @@ struct Buf {
 
 constexpr size_t LIST_PEEK = 1024;   // large-group list entries read back with the list counters
 
+// A pinned double buffer for host <-> device copies: the DMA of one chunk runs while CPU threads fill or drain the
+// other; ev[b] marks the last DMA that used buf[b].
+struct Pins {
+    uint8_t *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+// One combined batch of the host-pointer entry points in flight: its packed bytes on the device, its own pinned chunks
+// and copy stream, so that its host copies and DMA run while another batch's kernels do. staged: the inputs are on
+// the device; computed: the kernels are done.
+struct Lane {
+    std::mutex mu;
+    bool ready = false;
+    Buf T, U;
+    Pins pins;
+    hipStream_t cs = nullptr;
+    hipEvent_t staged = nullptr, computed = nullptr;
+};
+
+// Executable graphs of one launch sequence, by key (shapes and buffers). A key met on one of the last two calls is
+// captured when it comes again, and the two graphs used last are kept, so that calls alternating between two shapes
+// or two buffers (the two lanes of combined batches) both replay.
+struct GraphCache {
+    struct Entry {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t used = 0;
+    };
+    Entry e[2];
+    std::vector<uint64_t> seen[2];
+    uint64_t tick = 0;
+    void clear() {
+        for (Entry &x : e) {
+            if (x.exec) (void)hipGraphExecDestroy(x.exec);
+            x.exec = nullptr;
+            x.key.clear();
+            x.used = 0;
+        }
+        seen[0].clear();
+        seen[1].clear();
+    }
+};
+
 struct Workspace {
     int dev = -1;
     hipStream_t stream = nullptr;
@@ -146,14 +189,12 @@ struct Workspace {
     uint32_t *h_gcnt = nullptr;
     unsigned long long *orand = nullptr;    // 2
     uint32_t *m_out = nullptr;              // 1
-    // the inverse's launch sequence as an executable graph, for a call that repeats the previous call's shape
-    struct {
-        std::vector<uint64_t> key, last;
-        hipGraphExec_t exec = nullptr;
-    } inv_graph;
-    // pinned host staging
-    uint8_t *pin[2] = {nullptr, nullptr};   // the host-pointer entry points' double buffer (PIN_CHUNK bytes each)
-    hipEvent_t pin_ev[2] = {nullptr, nullptr};
+    // the inverse's launch sequence as executable graphs, for calls that repeat a recent call's shape
+    GraphCache inv_graph;
+    // pinned host staging: the host-pointer entry points' double buffer (PIN_CHUNK bytes each)
+    Pins pins;
+    // the combined batches of concurrent calls, one lane each (kbwt_combine.h); a lane's mutex is taken before mu
+    Lane lane[COMBINE_LANES];
     void *h_meta = nullptr;
     size_t h_meta_cap = 0;
     unsigned long long *h_orand = nullptr, *h_orand_init = nullptr;
@@ -274,11 +315,22 @@ struct Span {   // host bytes
 };
 CopyPool g_copy;
 
-void pin_ready(Workspace &w) {
-    if (w.pin[0]) return;
+void release_pins(Pins &pp) {
     for (int b = 0; b < 2; b++) {
-        HIPCHK(hipHostMalloc((void **)&w.pin[b], PIN_CHUNK, hipHostMallocDefault));
-        HIPCHK(hipEventCreateWithFlags(&w.pin_ev[b], hipEventDisableTiming));
+        if (pp.ev[b]) {
+            (void)hipEventSynchronize(pp.ev[b]);
+            (void)hipEventDestroy(pp.ev[b]);
+        }
+        if (pp.buf[b]) (void)hipHostFree(pp.buf[b]);
+        pp.buf[b] = nullptr;
+        pp.ev[b] = nullptr;
+    }
+}
+
+void pin_ready(Pins &pp) {
+    for (int b = 0; b < 2; b++) {
+        if (!pp.buf[b]) HIPCHK(hipHostMalloc((void **)&pp.buf[b], PIN_CHUNK, hipHostMallocDefault));
+        if (!pp.ev[b]) HIPCHK(hipEventCreateWithFlags(&pp.ev[b], hipEventDisableTiming));
     }
 }
 
@@ -309,7 +361,7 @@ size_t span_total(const std::vector<Span> &sp) {
 }
 
 // Host spans to dst on stream s (returns once the last chunk is queued; the spans may then be reused).
-void to_device(Workspace &w, uint8_t *dst, const std::vector<Span> &sp, hipStream_t s) {
+void to_device(Pins &pp, uint8_t *dst, const std::vector<Span> &sp, hipStream_t s) {
     fault_point(FS_COPY);
     const size_t n = span_total(sp);
     if (n < PIN_MIN) {
@@ -319,21 +371,21 @@ void to_device(Workspace &w, uint8_t *dst, const std::vector<Span> &sp, hipStrea
         }
         return;
     }
-    pin_ready(w);
+    pin_ready(pp);
     std::pair<size_t, size_t> at{0, 0};
     while (at.first < sp.size() && sp[at.first].n == 0) at.first++;
     for (size_t o = 0, k = 0; o < n; o += PIN_CHUNK, k++) {
         const size_t len = std::min(PIN_CHUNK, n - o);
         const int b = (int)(k & 1);
-        HIPCHK(hipEventSynchronize(w.pin_ev[b]));   // the DMA that read this buffer last is done
-        g_copy.copy(span_segs(sp, len, w.pin[b], true, at));
-        HIPCHK(hipMemcpyAsync(dst + o, w.pin[b], len, hipMemcpyHostToDevice, s));
-        HIPCHK(hipEventRecord(w.pin_ev[b], s));
+        HIPCHK(hipEventSynchronize(pp.ev[b]));   // the DMA that read this buffer last is done
+        g_copy.copy(span_segs(sp, len, pp.buf[b], true, at));
+        HIPCHK(hipMemcpyAsync(dst + o, pp.buf[b], len, hipMemcpyHostToDevice, s));
+        HIPCHK(hipEventRecord(pp.ev[b], s));
     }
 }
 
 // src to the host spans on stream s, after the work queued on s so far (returns with the bytes in the spans).
-void to_host(Workspace &w, const std::vector<Span> &sp, const uint8_t *src, hipStream_t s) {
+void to_host(Pins &pp, const std::vector<Span> &sp, const uint8_t *src, hipStream_t s) {
     fault_point(FS_COPY);
     const size_t n = span_total(sp);
     if (n < PIN_MIN) {
@@ -344,14 +396,14 @@ void to_host(Workspace &w, const std::vector<Span> &sp, const uint8_t *src, hipS
         SYNCCHK(s);
         return;
     }
-    pin_ready(w);
+    pin_ready(pp);
     const size_t chunks = (n + PIN_CHUNK - 1) / PIN_CHUNK;
     auto issue = [&](size_t k) {
         const int b = (int)(k & 1);
-        HIPCHK(hipEventSynchronize(w.pin_ev[b]));
-        HIPCHK(hipMemcpyAsync(w.pin[b], src + k * PIN_CHUNK, std::min(PIN_CHUNK, n - k * PIN_CHUNK),
+        HIPCHK(hipEventSynchronize(pp.ev[b]));
+        HIPCHK(hipMemcpyAsync(pp.buf[b], src + k * PIN_CHUNK, std::min(PIN_CHUNK, n - k * PIN_CHUNK),
                               hipMemcpyDeviceToHost, s));
-        HIPCHK(hipEventRecord(w.pin_ev[b], s));
+        HIPCHK(hipEventRecord(pp.ev[b], s));
     };
     std::pair<size_t, size_t> at{0, 0};
     while (at.first < sp.size() && sp[at.first].n == 0) at.first++;
@@ -359,8 +411,8 @@ void to_host(Workspace &w, const std::vector<Span> &sp, const uint8_t *src, hipS
     for (size_t k = 0; k < chunks; k++) {
         if (k + 1 < chunks) issue(k + 1);   // the other buffer: its chunk was copied out in the step before
         const int b = (int)(k & 1);
-        HIPCHK(hipEventSynchronize(w.pin_ev[b]));
-        g_copy.copy(span_segs(sp, std::min(PIN_CHUNK, n - k * PIN_CHUNK), w.pin[b], false, at));
+        HIPCHK(hipEventSynchronize(pp.ev[b]));
+        g_copy.copy(span_segs(sp, std::min(PIN_CHUNK, n - k * PIN_CHUNK), pp.buf[b], false, at));
     }
 }
 
@@ -387,38 +439,49 @@ bool prof_touches(std::initializer_list<const char *> names) {
 }
 
 // Run `body` (a fixed launch sequence on stream s, whose launches are `names`) directly, or from an executable
-// graph: a call whose key (shapes and buffers) equals the previous call's captures the sequence once and then
-// replays it. Graphs are skipped while profiling times one of the launches (events bracket them), on the legacy
-// default stream (it cannot be captured) and with KBWT_GRAPHS=0.
-template <class G, class F>
-void run_graph(G &g, const std::vector<uint64_t> &key, hipStream_t s, std::initializer_list<const char *> names,
-               F body) {
+// graph of g: a key met on one of the last two calls is captured when it comes again, and replayed from then on.
+// Graphs are skipped while profiling times one of the launches (events bracket them), on the legacy default stream
+// (it cannot be captured) and with KBWT_GRAPHS=0.
+template <class F>
+void run_graph(GraphCache &g, const std::vector<uint64_t> &key, hipStream_t s,
+               std::initializer_list<const char *> names, F body) {
     const bool eligible = s != nullptr && !prof_touches(names) && env_u32("KBWT_GRAPHS", 1, 0, 1);
-    if (!eligible || key != g.last) {
-        g.last = key;
+    if (!eligible) {
         body();
         return;
     }
-    if (!g.exec || g.key != key) {
-        if (g.exec) {
-            (void)hipGraphExecDestroy(g.exec);
-            g.exec = nullptr;
+    g.tick++;
+    for (GraphCache::Entry &x : g.e)
+        if (x.exec && x.key == key) {
+            x.used = g.tick;
+            HIPCHK(hipGraphLaunch(x.exec, s));
+            return;
         }
-        hipGraph_t graph = nullptr;
-        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        try {
-            body();
-        } catch (...) {   // leave the stream out of capture mode before the error goes on
-            if (hipStreamEndCapture(s, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
-            throw;
-        }
-        HIPCHK(hipStreamEndCapture(s, &graph));
-        const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        HIPCHK(e);
-        g.key = key;
+    if (key != g.seen[0] && key != g.seen[1]) {
+        g.seen[1].swap(g.seen[0]);
+        g.seen[0] = key;
+        body();
+        return;
     }
-    HIPCHK(hipGraphLaunch(g.exec, s));
+    GraphCache::Entry &x = g.e[0].used <= g.e[1].used ? g.e[0] : g.e[1];
+    if (x.exec) (void)hipGraphExecDestroy(x.exec);
+    x.exec = nullptr;
+    x.key.clear();
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        body();
+    } catch (...) {   // leave the stream out of capture mode before the error goes on
+        if (hipStreamEndCapture(s, &graph) == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(s, &graph));
+    const hipError_t e = hipGraphInstantiate(&x.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+    x.key = key;
+    x.used = g.tick;
+    HIPCHK(hipGraphLaunch(x.exec, s));
 }
 
 // Launch `fn` on stream s; with profiling on, bracket it with a pair of events.
@@ -1434,11 +1497,8 @@ void recover(Workspace &w, hipStream_t s) {
     (void)hipStreamSynchronize(s);
     if (s != w.stream) (void)hipStreamSynchronize(w.stream);
     for (int b = 0; b < 2; b++)
-        if (w.pin_ev[b]) (void)hipEventSynchronize(w.pin_ev[b]);
-    if (w.inv_graph.exec) (void)hipGraphExecDestroy(w.inv_graph.exec);
-    w.inv_graph.exec = nullptr;
-    w.inv_graph.key.clear();
-    w.inv_graph.last.clear();
+        if (w.pins.ev[b]) (void)hipEventSynchronize(w.pins.ev[b]);
+    w.inv_graph.clear();
     try {
         for (auto &p : w.pending) {
             w.spare.push_back(p.a);
@@ -1520,7 +1580,28 @@ static int32_t cpu_fallback(bool fwd, const uint8_t *T, uint8_t *U, int32_t n, i
 // buffers packed back to back in the workspace's staging buffers
 static Combiner g_combiner;
 
-static void run_pending_batch(std::vector<Pending *> &b, bool fwd) {
+// A lane's stream and events, made on its first batch.
+static void lane_ready(Lane &ln) {
+    if (ln.ready) return;
+    if (!ln.cs) HIPCHK(hipStreamCreateWithFlags(&ln.cs, hipStreamNonBlocking));
+    if (!ln.staged) HIPCHK(hipEventCreateWithFlags(&ln.staged, hipEventDisableTiming));
+    if (!ln.computed) HIPCHK(hipEventCreateWithFlags(&ln.computed, hipEventDisableTiming));
+    ln.ready = true;
+}
+
+// After an error in a lane's batch: let its queued copies finish and clear the runtime's last error. Never throws.
+static void recover_lane(Lane &ln) {
+    if (ln.cs) (void)hipStreamSynchronize(ln.cs);
+    for (int b = 0; b < 2; b++)
+        if (ln.pins.ev[b]) (void)hipEventSynchronize(ln.pins.ev[b]);
+    (void)hipGetLastError();
+}
+
+// One combined batch on lane `lane`, in three steps: the callers' bytes to the lane's device buffer on the lane's copy
+// stream (no workspace lock, so another batch's kernels run meanwhile), then in.done() and the kernels on the
+// workspace's stream once the copy is in (workspace lock), then the results back to the callers' buffers on the
+// lane's stream, while the next batch's kernels run.
+static void run_pending_batch(std::vector<Pending *> &b, bool fwd, int lane, Intake &in) {
     const int32_t count = (int32_t)b.size();
     std::vector<kbwt_seg> segs(count);
     int64_t at = 0;
@@ -1536,18 +1617,52 @@ static void run_pending_batch(std::vector<Pending *> &b, bool fwd) {
     if (!fwd)
         for (int32_t k = 0, o = 0; k < count; o += (int32_t)nsamples(b[k]->n, b[k]->r), k++)
             memcpy(I.data() + o, b[k]->Iin, nsamples(b[k]->n, b[k]->r) * 4);
-    const int32_t rc = on_device(nullptr, [&](Workspace &w, hipStream_t s) {
-        grow(w.T, (size_t)at + 64);
-        if (fwd) grow(w.U, (size_t)at + 64);
-        std::vector<Span> sp(count);
-        for (int32_t k = 0; k < count; k++) sp[k] = {const_cast<uint8_t *>(b[k]->T), (size_t)b[k]->n};
-        to_device(w, P<uint8_t>(w.T), sp, s);
-        uint8_t *out = fwd ? P<uint8_t>(w.U) : P<uint8_t>(w.T);
-        if (fwd) forward_batch_dev(w, P<uint8_t>(w.T), out, segs.data(), count, I.data(), s);
-        else inverse_batch_dev(w, P<uint8_t>(w.T), out, segs.data(), count, I.data(), s);
-        for (int32_t k = 0; k < count; k++) sp[k].p = b[k]->U;
-        to_host(w, sp, out, s);
+    std::vector<Span> sp(count);
+    for (int32_t k = 0; k < count; k++) sp[k] = {const_cast<uint8_t *>(b[k]->T), (size_t)b[k]->n};
+    Lane *lp = nullptr;
+    int32_t rc = guarded([&] {
+        Workspace &w = workspace();
+        lp = &w.lane[lane];
     });
+    std::unique_lock<std::mutex> lane_lk;
+    if (rc == 0) lane_lk = std::unique_lock<std::mutex>(lp->mu);
+    uint8_t *out = nullptr;
+    if (rc == 0)
+        rc = guarded([&] {
+            Lane &ln = *lp;
+            try {
+                lane_ready(ln);
+                grow(ln.T, (size_t)at + 64);
+                if (fwd) grow(ln.U, (size_t)at + 64);
+                out = fwd ? P<uint8_t>(ln.U) : P<uint8_t>(ln.T);
+                to_device(ln.pins, P<uint8_t>(ln.T), sp, ln.cs);
+                HIPCHK(hipEventRecord(ln.staged, ln.cs));
+            } catch (...) {
+                recover_lane(ln);
+                throw;
+            }
+        });
+    in.done();
+    if (rc == 0)
+        rc = on_device(nullptr, [&](Workspace &w, hipStream_t s) {
+            HIPCHK(hipStreamWaitEvent(s, lp->staged, 0));
+            if (fwd) forward_batch_dev(w, P<uint8_t>(lp->T), out, segs.data(), count, I.data(), s);
+            else inverse_batch_dev(w, P<uint8_t>(lp->T), out, segs.data(), count, I.data(), s);
+            HIPCHK(hipEventRecord(lp->computed, s));
+        });
+    if (rc == 0)
+        rc = guarded([&] {
+            Lane &ln = *lp;
+            try {
+                HIPCHK(hipStreamWaitEvent(ln.cs, ln.computed, 0));
+                for (int32_t k = 0; k < count; k++) sp[k].p = b[k]->U;
+                to_host(ln.pins, sp, out, ln.cs);
+            } catch (...) {
+                recover_lane(ln);
+                throw;
+            }
+        });
+    if (rc != 0 && lp) recover_lane(*lp);
     const std::string err = g_last_error;
     for (int32_t k = 0, o = 0; k < count; o += (int32_t)nsamples(b[k]->n, b[k]->r), k++) {
         b[k]->rc = rc;
@@ -1558,10 +1673,10 @@ static void run_pending_batch(std::vector<Pending *> &b, bool fwd) {
 
 // Never throws: a caller that runs a batch must always wake the others. A block that fails on its own (the combiner
 // runs the blocks of a failed batch again one by one) takes the CPU path when it is enabled.
-static void run_pending(std::vector<Pending *> &b, bool fwd) {
+static void run_pending(std::vector<Pending *> &b, bool fwd, int lane, Intake &in) {
     if (b.empty()) return;
     try {
-        run_pending_batch(b, fwd);
+        run_pending_batch(b, fwd, lane, in);
     } catch (...) {
         for (Pending *q : b) {
             q->rc = KBWT_ERR_NOMEM;
@@ -1868,11 +1983,11 @@ int32_t kbwt_bwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs, i
         fault_point(FS_ALLOC);
         HIPCHK(hipMalloc(&in.p, (size_t)(hi - lo) + 64));
         HIPCHK(hipMalloc(&out.p, (size_t)(hi - lo) + 64));
-        to_device(w, (uint8_t *)in.p, {{const_cast<uint8_t *>(T) + lo, (size_t)(hi - lo)}}, s);
+        to_device(w.pins, (uint8_t *)in.p, {{const_cast<uint8_t *>(T) + lo, (size_t)(hi - lo)}}, s);
         std::vector<kbwt_seg> g(segs, segs + count);
         for (auto &x : g) x.offset -= lo;
         forward_batch_dev(w, (const uint8_t *)in.p, (uint8_t *)out.p, g.data(), count, I, s);
-        to_host(w, {{U + lo, (size_t)(hi - lo)}}, (const uint8_t *)out.p, s);
+        to_host(w.pins, {{U + lo, (size_t)(hi - lo)}}, (const uint8_t *)out.p, s);
         SYNCCHK(s);
     });
     if (rc == KBWT_OK || !fallback_on()) return rc;
@@ -1893,11 +2008,11 @@ int32_t kbwt_unbwt_aux_batch(const uint8_t *T, uint8_t *U, const kbwt_seg *segs,
         DevSpan io;
         fault_point(FS_ALLOC);
         HIPCHK(hipMalloc(&io.p, (size_t)(hi - lo) + 64));
-        to_device(w, (uint8_t *)io.p, {{const_cast<uint8_t *>(T) + lo, (size_t)(hi - lo)}}, s);
+        to_device(w.pins, (uint8_t *)io.p, {{const_cast<uint8_t *>(T) + lo, (size_t)(hi - lo)}}, s);
         std::vector<kbwt_seg> g(segs, segs + count);
         for (auto &x : g) x.offset -= lo;
         inverse_batch_dev(w, (const uint8_t *)io.p, (uint8_t *)io.p, g.data(), count, I, s);
-        to_host(w, {{U + lo, (size_t)(hi - lo)}}, (const uint8_t *)io.p, s);
+        to_host(w.pins, {{U + lo, (size_t)(hi - lo)}}, (const uint8_t *)io.p, s);
         SYNCCHK(s);
     });
     if (rc == KBWT_OK || !fallback_on()) return rc;
@@ -1998,32 +2113,32 @@ void kbwt_release(void) {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
     Workspace &w = g_ws[dev];
+    // the lanes first (a batch holds its lane, then the workspace), so that no batch in flight loses its buffers
+    std::unique_lock<std::mutex> lanes[COMBINE_LANES];
+    for (int k = 0; k < COMBINE_LANES; k++) lanes[k] = std::unique_lock<std::mutex>(w.lane[k].mu);
     std::lock_guard<std::mutex> lk(w.mu);
+    for (Lane &ln : w.lane) {
+        if (ln.cs) (void)hipStreamSynchronize(ln.cs);
+        for (Buf *b : {&ln.T, &ln.U}) {
+            if (b->p) (void)hipFree(b->p);
+            b->p = nullptr;
+            b->cap = 0;
+        }
+        release_pins(ln.pins);
+    }
     Buf *bufs[] = {&w.T,       &w.U,      &w.K0,    &w.K1,     &w.V0,     &w.V1,     &w.SA,     &w.ISA,    &w.L,
                    &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.carry, &w.samples,
                    &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
                    &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
                    &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,
                    &w.sufl,    &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax,   &w.lrow};
-    if (w.inv_graph.exec) {
-        (void)hipGraphExecDestroy(w.inv_graph.exec);
-        w.inv_graph.exec = nullptr;
-    }
-    w.inv_graph.key.clear();
-    w.inv_graph.last.clear();
+    w.inv_graph.clear();
     for (Buf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;
         b->cap = 0;
     }
-    for (int b = 0; b < 2; b++) {
-        if (!w.pin[b]) continue;
-        (void)hipEventSynchronize(w.pin_ev[b]);
-        (void)hipEventDestroy(w.pin_ev[b]);
-        (void)hipHostFree(w.pin[b]);
-        w.pin[b] = nullptr;
-        w.pin_ev[b] = nullptr;
-    }
+    release_pins(w.pins);
 }
 
 }  // extern "C"

@@ -2,7 +2,9 @@
 // The runner's "forward" writes each byte plus one, its "inverse" each byte minus one. A batch that holds a poisoned
 // block (first byte 0xEE) fails as a whole with -3, as a device error would; run alone, the poisoned block still
 // fails and every other block succeeds. Checks: every caller gets its own block's bytes and code, failures stay with
-// the poisoned blocks, a block longer than a batch may hold is refused without a hang, and batching happens.
+// the poisoned blocks, a block longer than a batch may hold is refused without a hang, and batching happens. Lanes:
+// no lane carries two batches at once, at most one batch stages its inputs at a time (before Intake::done()), and two
+// batches do run at once (one staging while the other computes).
 #include "kbwt_combine.h"
 
 #include <atomic>
@@ -15,11 +17,19 @@
 int main(int argc, char **argv) {
     const int iters = argc > 1 ? atoi(argv[1]) : 100;
     kbwt::Combiner comb;
-    std::atomic<int> bad{0}, poisoned_seen{0}, multi{0};
-    auto runner = [&](std::vector<kbwt::Pending *> &b, bool fwd) {
+    std::atomic<int> bad{0}, poisoned_seen{0}, multi{0}, staging{0}, running{0}, overlapped{0};
+    std::atomic<int> lane_busy[kbwt::COMBINE_LANES] = {};
+    auto runner = [&](std::vector<kbwt::Pending *> &b, bool fwd, int lane, kbwt::Intake &in) {
+        if (lane < 0 || lane >= kbwt::COMBINE_LANES || lane_busy[lane]++ != 0) bad++;
+        if (running++ > 0) overlapped++;
         if (b.size() > 1) multi++;
         bool poison = false;
         for (auto *q : b) poison |= q->n > 0 && q->T[0] == 0xEE;
+        const bool first = !in.staged();   // a batch's first run, not a block run again after its batch failed
+        if (first && staging++ != 0) bad++;   // the "inputs": only one batch stages at a time
+        std::this_thread::sleep_for(std::chrono::microseconds(100));
+        if (first) staging--;
+        in.done();
         std::this_thread::sleep_for(std::chrono::microseconds(200));
         for (auto *q : b) {
             if (poison) {
@@ -31,6 +41,8 @@ int main(int argc, char **argv) {
             if (fwd) q->I[0] = q->n;
             q->rc = 0;
         }
+        running--;
+        lane_busy[lane]--;
     };
     std::vector<std::thread> th;
     for (int c = 0; c < 12; c++)
@@ -74,10 +86,11 @@ int main(int argc, char **argv) {
     for (auto &t : th) t.join();
     uint64_t batches = 0, blocks = 0, retried = 0;
     comb.counts(batches, blocks, retried);
-    printf("combine: %llu batches for %llu blocks, %d multi-block runs, %d poisoned calls, %llu retried, %d bad\n",
+    printf("combine: %llu batches for %llu blocks, %d multi-block runs, %d poisoned calls, %llu retried, %d runs "
+           "beside another, %d bad\n",
            (unsigned long long)batches, (unsigned long long)blocks, multi.load(), poisoned_seen.load(),
-           (unsigned long long)retried, bad.load());
-    if (bad || multi == 0 || poisoned_seen == 0 || retried == 0) return 1;
+           (unsigned long long)retried, overlapped.load(), bad.load());
+    if (bad || multi == 0 || poisoned_seen == 0 || retried == 0 || overlapped == 0) return 1;
     printf("combine ok\n");
     return 0;
 }
