@@ -642,6 +642,16 @@ void router_features_tail(Workspace &w, const uint8_t *dT, const BlockInfo *d_bl
     }
 }
 
+// A radix pass, after checking on the host that its grid fits what the kernel assumes: the tiles were counted with the
+// configuration's own tile size, and a flat pass (no per-block tiling) has exactly ceil(m / tile) of them, so that no
+// tile starts past the keys.
+void sort_pass_checked(const SortArgs &a, uint32_t ntiles, uint32_t tile, hipStream_t s) {
+    if (tile != sort_tile(a.cfg) ||
+        (!a.tiling.first && (uint64_t)ntiles != ((uint64_t)a.m + tile - 1) / tile))
+        fail(hipErrorInvalidValue, "radix pass launch does not match its configuration's tile");
+    launch_sort_pass(a, ntiles, s);
+}
+
 // bl: one BlockInfo per block with off, n (>= 2), r and soff filled in; packed input dT (N bytes), output dU.
 // I: host array of nsamp sampled indexes. Per-call metadata (blocks, tile tables, code tables) goes to the device
 // in one copy.
@@ -778,7 +788,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         a.epoch = next_epoch(w, s);
         a.T = dT;
         a.codes = d_codes;
-        TIMED(p == 0 ? "sort_pass_T" : "sort_pass", N, launch_sort_pass(a, nst, s));
+        TIMED(p == 0 ? "sort_pass_T" : "sort_pass", N, sort_pass_checked(a, nst, stile, s));
         LAUNCHCHK();
     }
     st.passes[0] = (int32_t)P0;
@@ -976,7 +986,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
                     a.status = P<uint64_t>(w.status);
                     a.counter = w.counters + 0;
                     a.epoch = next_epoch(w, s);
-                    TIMED("sort_pass", ml, launch_sort_pass(a, ntl, s));
+                    TIMED("sort_pass", ml, sort_pass_checked(a, ntl, stile, s));
                     LAUNCHCHK();
                     src ^= 1;
                 }

@@ -311,7 +311,7 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
     const uint32_t tile = s_tile;
     const TileRange tr = tile_range(tl, tile, TILE, m);
     const size_t base = tr.beg;
-    const uint32_t cnt = (uint32_t)(tr.end - tr.beg);
+    const uint32_t cnt = tr.end > tr.beg ? (uint32_t)(tr.end - tr.beg) : 0u;   // a tile past the keys holds none
     const uint32_t *db = dbase ? dbase + (size_t)tr.b * dstride : nullptr;
     uint32_t woff = 0, wn = 0, wsh = 0;   // keys-only passes: the block's range and window width
     if (!VALS) {
