@@ -209,6 +209,7 @@ typedef struct {
     int64_t injected_faults;   /* faults injected by kbwt_fault_inject / KBWT_FAULT */
     int64_t combined_batches;  /* batches of concurrent kolmo_sais_* calls */
     int64_t combined_blocks;   /* blocks those batches held */
+    int64_t graph_replays;     /* launch sequences run from an executable graph (the forward's first sort, the inverse) */
 } kbwt_counters;
 KBWT_API int32_t kbwt_set_cpu_fallback(int32_t on);
 KBWT_API int32_t kbwt_fault_inject(const char *spec);

@@ -55,7 +55,8 @@ class Counters(ctypes.Structure):
     """kbwt_counters: process-wide counts of the error path and of combined batches."""
     _fields_ = [("device_errors", ctypes.c_int64), ("retried_blocks", ctypes.c_int64),
                 ("cpu_fallbacks", ctypes.c_int64), ("injected_faults", ctypes.c_int64),
-                ("combined_batches", ctypes.c_int64), ("combined_blocks", ctypes.c_int64)]
+                ("combined_batches", ctypes.c_int64), ("combined_blocks", ctypes.c_int64),
+                ("graph_replays", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

@@ -66,7 +66,7 @@ struct FaultPlan {
 };
 FaultPlan g_fault;
 struct Counters {
-    std::atomic<int64_t> device_errors{0}, retried_blocks{0}, cpu_fallbacks{0}, injected_faults{0};
+    std::atomic<int64_t> device_errors{0}, retried_blocks{0}, cpu_fallbacks{0}, injected_faults{0}, graph_replays{0};
 };
 Counters g_counters;
 std::atomic<int> g_cpu_fallback{-1};   // -1: not read yet from KBWT_CPU_FALLBACK
@@ -189,8 +189,9 @@ struct Workspace {
     uint32_t *h_gcnt = nullptr;
     unsigned long long *orand = nullptr;    // 2
     uint32_t *m_out = nullptr;              // 1
-    // the inverse's launch sequence as executable graphs, for calls that repeat a recent call's shape
-    GraphCache inv_graph;
+    // the inverse's launch sequence, and the forward's first sort, as executable graphs, for calls that repeat a
+    // recent call's shape
+    GraphCache inv_graph, fwd_graph;
     // pinned host staging: the host-pointer entry points' double buffer (PIN_CHUNK bytes each)
     Pins pins;
     // the combined batches of concurrent calls, one lane each (kbwt_combine.h); a lane's mutex is taken before mu
@@ -455,6 +456,7 @@ void run_graph(GraphCache &g, const std::vector<uint64_t> &key, hipStream_t s,
         if (x.exec && x.key == key) {
             x.used = g.tick;
             HIPCHK(hipGraphLaunch(x.exec, s));
+            g_counters.graph_replays++;
             return;
         }
     if (key != g.seen[0] && key != g.seen[1]) {
@@ -482,6 +484,7 @@ void run_graph(GraphCache &g, const std::vector<uint64_t> &key, hipStream_t s,
     x.key = key;
     x.used = g.tick;
     HIPCHK(hipGraphLaunch(x.exec, s));
+    g_counters.graph_replays++;
 }
 
 // Launch `fn` on stream s; with profiling on, bracket it with a pair of events.
@@ -759,43 +762,50 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         launch_fill_u8(P<uint8_t>(w.act), 0, (size_t)ROUTER_ROUNDS * B * 4, s);
     }
 
-    // 2. digit histograms of the first keys
-    TIMED("hist_T", N, launch_hist_T(dT, thist, nht, d_codes, P0, P<uint32_t>(w.counts), s));
-    TIMED("scan_digits", 256 * P0 * B,
-          launch_scan_digits(P<uint32_t>(w.counts), B * P0, P0, d_bl, 1, nullptr, P<uint32_t>(w.dbase), s));
-    LAUNCHCHK();
-
-    // 3. the first sort: P0 stable passes per block over the digits above the BWT byte; the last writes the SA
+    // 2-4. The digit histograms of the first keys, the first sort, the groups after it and the ISA: a launch sequence
+    // that depends only on the shapes and buffers below (the data travel in the metadata buffer), so that a call
+    // repeating a recent call's shape replays it from an executable graph. Its lookbacks take their epochs from the
+    // device epoch, which the sequence moves on first.
     uint64_t *K[2] = {P<uint64_t>(w.K0), P<uint64_t>(w.K1)};
     uint32_t *V[2] = {P<uint32_t>(w.V0), P<uint32_t>(w.V1)};
     uint32_t *SA = P<uint32_t>(w.SA), *ISA = P<uint32_t>(w.ISA);
     uint8_t *L = P<uint8_t>(w.L);
-    for (uint32_t p = 0; p < P0; p++) {
-        SortArgs a{};
-        a.cfg = scfg;
-        a.from_t = p == 0;
-        a.kin = p ? K[(p - 1) & 1] : nullptr;
-        a.vin = p ? V[(p - 1) & 1] : nullptr;
-        a.kout = K[p & 1];
-        a.vout = p == P0 - 1 ? SA : V[p & 1];
-        a.m = N;
-        a.shift = 8 * p + 8;
-        a.dbase = P<uint32_t>(w.dbase) + p * 256;
-        a.dstride = P0 * 256;
-        a.tiling = tsort;
-        a.status = P<uint64_t>(w.status);
-        a.counter = w.counters + 0;
-        a.epoch = next_epoch(w, s);
-        a.T = dT;
-        a.codes = d_codes;
-        TIMED(p == 0 ? "sort_pass_T" : "sort_pass", N, sort_pass_checked(a, nst, stile, s));
-        LAUNCHCHK();
+    uint32_t *depoch = w.counters + 15;
+    if (++w.dev_epoch_calls >= EPOCH_HALF / EPOCH_STRIDE - 1) {   // the device epoch is about to come round again
+        clear_status(w, s);
+        w.dev_epoch_calls = 0;
     }
-    st.passes[0] = (int32_t)P0;
-    st.active[0] = N;
-
-    // 4. groups after the first sort; the BWT byte of every row is its key's low byte
-    {
+    auto first_sort = [&] {
+        launch_epoch_bump(depoch, s);
+        // 2. digit histograms of the first keys
+        TIMED("hist_T", N, launch_hist_T(dT, thist, nht, d_codes, P0, P<uint32_t>(w.counts), s));
+        TIMED("scan_digits", 256 * P0 * B,
+              launch_scan_digits(P<uint32_t>(w.counts), B * P0, P0, d_bl, 1, nullptr, P<uint32_t>(w.dbase), s));
+        LAUNCHCHK();
+        // 3. the first sort: P0 stable passes per block over the digits above the BWT byte; the last writes the SA
+        for (uint32_t p = 0; p < P0; p++) {
+            SortArgs a{};
+            a.cfg = scfg;
+            a.from_t = p == 0;
+            a.kin = p ? K[(p - 1) & 1] : nullptr;
+            a.vin = p ? V[(p - 1) & 1] : nullptr;
+            a.kout = K[p & 1];
+            a.vout = p == P0 - 1 ? SA : V[p & 1];
+            a.m = N;
+            a.shift = 8 * p + 8;
+            a.dbase = P<uint32_t>(w.dbase) + p * 256;
+            a.dstride = P0 * 256;
+            a.tiling = tsort;
+            a.status = P<uint64_t>(w.status);
+            a.counter = w.counters + 0;
+            a.epoch = p;              // offsets 0..P0-1 from the device epoch; the partition below takes MAXP
+            a.depoch = depoch;
+            a.T = dT;
+            a.codes = d_codes;
+            TIMED(p == 0 ? "sort_pass_T" : "sort_pass", N, sort_pass_checked(a, nst, stile, s));
+            LAUNCHCHK();
+        }
+        // 4. groups after the first sort; the BWT byte of every row is its key's low byte
         RegroupArgs g{};
         g.K = K[(P0 - 1) & 1];
         g.V = SA;
@@ -821,12 +831,27 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         // ISA[suffix] = head: partition the pairs by suffix window inside each block (one keys-only radix pass),
         // then write window by window instead of scattering 4-byte writes over all of ISA
         TIMED("isa_part", N,
-              launch_partition_pairs(K[P0 & 1], K[(P0 - 1) & 1], N, tpart, npt, P<uint64_t>(w.status), w.counters + 4,
-                                     next_epoch(w, s), s));
+              launch_partition_pairs(K[P0 & 1], K[(P0 - 1) & 1], N, tpart, npt, P<uint64_t>(w.status),
+                                     w.counters + 4, MAXP, depoch, s));
         LAUNCHCHK();
         TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, ISA, s));
+        LAUNCHCHK();
+    };
+    if (feat) {   // the router's path: its per-round counts are set up around this sequence, so no graph
+        first_sort();
+    } else {
+        const std::vector<uint64_t> key = {
+            B, N, P0, (uint64_t)scfg, nht, nst, npt, (uintptr_t)dT, (uintptr_t)dm, (uintptr_t)w.counts.p,
+            (uintptr_t)w.dbase.p, (uintptr_t)w.K0.p, (uintptr_t)w.K1.p, (uintptr_t)w.V0.p, (uintptr_t)w.V1.p,
+            (uintptr_t)w.SA.p, (uintptr_t)w.ISA.p, (uintptr_t)w.L.p, (uintptr_t)w.carry.p, (uintptr_t)w.gh[0].p,
+            (uintptr_t)w.gs[0].p, (uintptr_t)w.lh[0].p, (uintptr_t)w.ls[0].p, (uintptr_t)w.status.p,
+            (uintptr_t)w.counters, (uintptr_t)w.gcnt, (uintptr_t)blk};
+        run_graph(w.fwd_graph, key, s,
+                  {"hist_T", "scan_digits", "sort_pass_T", "sort_pass", "regroup", "isa_part", "isa_scatter"},
+                  first_sort);
     }
-    LAUNCHCHK();
+    st.passes[0] = (int32_t)P0;
+    st.active[0] = N;
 
     // 5. doubling rounds over the groups of two or more (SA ranges). Small groups sort inside one workgroup each
     // (k_seg_round); groups of more than SEG_GROUP rows take the global path (round keys, radix passes, regroup).
@@ -1509,6 +1534,7 @@ void recover(Workspace &w, hipStream_t s) {
     for (int b = 0; b < 2; b++)
         if (w.pins.ev[b]) (void)hipEventSynchronize(w.pins.ev[b]);
     w.inv_graph.clear();
+    w.fwd_graph.clear();
     try {
         for (auto &p : w.pending) {
             w.spare.push_back(p.a);
@@ -2058,6 +2084,7 @@ int32_t kbwt_get_counters(kbwt_counters *out) {
     out->injected_faults = g_counters.injected_faults.load();
     out->combined_batches = (int64_t)batches;
     out->combined_blocks = (int64_t)blocks;
+    out->graph_replays = g_counters.graph_replays.load();
     return KBWT_OK;
 }
 
@@ -2143,6 +2170,7 @@ void kbwt_release(void) {
                    &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,
                    &w.sufl,    &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax,   &w.lrow};
     w.inv_graph.clear();
+    w.fwd_graph.clear();
     for (Buf *b : bufs) {
         if (b->p) (void)hipFree(b->p);
         b->p = nullptr;

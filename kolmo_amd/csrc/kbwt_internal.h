@@ -72,7 +72,8 @@ struct SortArgs {
     Tiling tiling;                // per-block segmented sort when tiling.first != nullptr
     uint64_t *status;
     uint32_t *counter;
-    uint32_t epoch;
+    uint32_t epoch;               // the lookback epoch, or its offset from *depoch when depoch is set
+    const uint32_t *depoch;       // device epoch of a launch sequence a graph may replay (nullptr: host epochs)
     bool from_t;                  // first pass: keys from T with each block's code table
     const uint8_t *T, *codes;
 };
@@ -129,7 +130,8 @@ void launch_sort_pass(const SortArgs &a, uint32_t ntiles, hipStream_t s);
 void launch_regroup(const RegroupArgs &a, hipStream_t s);
 constexpr int PART_BLOCK = 1024, PART_ITEMS = 16, PART_TILE = PART_BLOCK * PART_ITEMS;   // the ISA partition pass
 void launch_partition_pairs(const uint64_t *in, uint64_t *out, uint32_t N, const Tiling &tl, uint32_t ntiles,
-                            uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s);
+                            uint64_t *status, uint32_t *counter, uint32_t epoch, const uint32_t *depoch,
+                            hipStream_t s);
 void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t *ISA, hipStream_t s);
 void launch_round_keys(const uint32_t *pos, const uint32_t *grp, uint32_t m, const uint32_t *SA, const uint32_t *ISA,
                        uint32_t n, const uint16_t *blk, const BlockInfo *blocks, uint32_t round, uint64_t *K, uint32_t *V,

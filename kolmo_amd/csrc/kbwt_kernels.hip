@@ -276,6 +276,8 @@ __global__ __launch_bounds__(256) void k_hist_keys(const uint64_t *__restrict__ 
 // FROM_T: keys are built from T with the block's code table (value = the suffix index); this pass and the keys-only
 // one below need not be stable and are not. dbase[b*dstride + d]: start of
 // digit d in block b. status: one lookback word per (tile, digit); counter: tile-id counter.
+// epoch_arg: the lookback epoch, or with depoch its offset from the device epoch *depoch (a replayed graph reads its
+// epoch there).
 // Keys-only passes (VALS == false) move (suffix << 32 | head) pairs alone, partitioned by the suffix's window inside
 // its block b: digit (suffix - off_b) >> window_shift(n_b). The suffixes of a block are a permutation of its own
 // index range, so window d starts at off_b + min(d << window_shift(n_b), n_b) (dbase unused).
@@ -284,9 +286,11 @@ __global__ __launch_bounds__(BLOCK) void k_sort_pass(const uint64_t *__restrict_
                                                      uint64_t *__restrict__ kout, uint32_t *__restrict__ vout,
                                                      uint32_t m, uint32_t shift, const uint32_t *__restrict__ dbase,
                                                      uint32_t dstride, Tiling tl, uint64_t *__restrict__ status,
-                                                     uint32_t *__restrict__ counter, uint32_t epoch, uint32_t ntiles,
+                                                     uint32_t *__restrict__ counter, uint32_t epoch_arg,
+                                                     const uint32_t *__restrict__ depoch, uint32_t ntiles,
                                                      const uint8_t *__restrict__ T,
                                                      const uint8_t *__restrict__ codes) {
+    const uint32_t epoch = depoch ? *depoch + epoch_arg : epoch_arg;
     constexpr int W = BLOCK / WAVE;
     constexpr int TILE = BLOCK * ITEMS;
     static_assert(BLOCK >= 256, "one thread per digit");
@@ -1148,11 +1152,12 @@ static void sort_pass_cfg(const SortArgs &a, uint32_t ntiles, hipStream_t s) {
     if (a.from_t)
         k_sort_pass<BLOCK, ITEMS, LB, true><<<ntiles, BLOCK, 0, s>>>(a.kin, a.vin, a.kout, a.vout, a.m, a.shift, a.dbase,
                                                                     a.dstride, a.tiling, a.status, a.counter, a.epoch,
-                                                                    ntiles, a.T, a.codes);
+                                                                    a.depoch, ntiles, a.T, a.codes);
     else
         k_sort_pass<BLOCK, ITEMS, LB, false><<<ntiles, BLOCK, 0, s>>>(a.kin, a.vin, a.kout, a.vout, a.m, a.shift,
                                                                      a.dbase, a.dstride, a.tiling, a.status,
-                                                                     a.counter, a.epoch, ntiles, a.T, a.codes);
+                                                                     a.counter, a.epoch, a.depoch, ntiles, a.T,
+                                                                     a.codes);
 }
 void launch_sort_pass(const SortArgs &a, uint32_t ntiles, hipStream_t s) {
     if (!ntiles) return;
@@ -1171,10 +1176,11 @@ void launch_sort_pass(const SortArgs &a, uint32_t ntiles, hipStream_t s) {
     }
 }
 void launch_partition_pairs(const uint64_t *in, uint64_t *out, uint32_t N, const Tiling &tl, uint32_t ntiles,
-                            uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s) {
+                            uint64_t *status, uint32_t *counter, uint32_t epoch, const uint32_t *depoch,
+                            hipStream_t s) {
     if (!ntiles) return;
     k_sort_pass<PART_BLOCK, PART_ITEMS, 4, false, false><<<ntiles, PART_BLOCK, 0, s>>>(
-        in, nullptr, out, nullptr, N, 0, nullptr, 0, tl, status, counter, epoch, ntiles, nullptr, nullptr);
+        in, nullptr, out, nullptr, N, 0, nullptr, 0, tl, status, counter, epoch, depoch, ntiles, nullptr, nullptr);
 }
 void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t *ISA, hipStream_t s) {
     if (!N) return;
