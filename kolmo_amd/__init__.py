@@ -148,6 +148,8 @@ def workers_lib():
         W = ctypes.CDLL(WORKERS_LIB)
         W.kbwt_workers_run.restype = ctypes.c_double
         W.kbwt_workers_run.argtypes = [_u8p, ctypes.c_int64, ctypes.c_int32, _i32p, ctypes.c_int32, ctypes.c_int32]
+        W.kbwt_workers_last_error.restype = ctypes.c_char_p
+        W.kbwt_workers_last_error.argtypes = []
         _workers = W
     return _workers
 
@@ -163,7 +165,7 @@ def block_workers(T, block, workers, reps=1):
     sec = workers_lib().kbwt_workers_run(_ptr(T), T.size, block, _ptr(rs, _i32p), workers, reps)
     if sec < 0:
         raise KbwtError(int(sec), "block workers: bad arguments" if sec == -1 else
-                        "block workers: a call failed or a round trip differs from its block")
+                        "block workers: " + workers_lib().kbwt_workers_last_error().decode())
     return sec
 
 

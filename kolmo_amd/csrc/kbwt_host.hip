@@ -442,11 +442,19 @@ bool prof_touches(std::initializer_list<const char *> names) {
 // Run `body` (a fixed launch sequence on stream s, whose launches are `names`) directly, or from an executable
 // graph of g: a key met on one of the last two calls is captured when it comes again, and replayed from then on.
 // Graphs are skipped while profiling times one of the launches (events bracket them), on the legacy default stream
-// (it cannot be captured) and with KBWT_GRAPHS=0.
+// (it cannot be captured), for combined batches (GraphsOff) and with KBWT_GRAPHS=0.
+// GraphsOff: while one lives on a thread, that thread's launch sequences run directly (no capture, no replay).
+thread_local int g_graphs_off = 0;
+struct GraphsOff {
+    GraphsOff() { g_graphs_off++; }
+    ~GraphsOff() { g_graphs_off--; }
+};
+
 template <class F>
 void run_graph(GraphCache &g, const std::vector<uint64_t> &key, hipStream_t s,
                std::initializer_list<const char *> names, F body) {
-    const bool eligible = s != nullptr && !prof_touches(names) && env_u32("KBWT_GRAPHS", 1, 0, 1);
+    const bool eligible =
+        s != nullptr && !g_graphs_off && !prof_touches(names) && env_u32("KBWT_GRAPHS", 1, 0, 1);
     if (!eligible) {
         body();
         return;
@@ -1681,6 +1689,9 @@ static void run_pending_batch(std::vector<Pending *> &b, bool fwd, int lane, Int
     in.done();
     if (rc == 0)
         rc = on_device(nullptr, [&](Workspace &w, hipStream_t s) {
+            // no graph capture here: combined batches seldom repeat a shape, and other callers' threads allocate
+            // and copy meanwhile (their lanes' staging)
+            GraphsOff off;
             HIPCHK(hipStreamWaitEvent(s, lp->staged, 0));
             if (fwd) forward_batch_dev(w, P<uint8_t>(lp->T), out, segs.data(), count, I.data(), s);
             else inverse_batch_dev(w, P<uint8_t>(lp->T), out, segs.data(), count, I.data(), s);
@@ -1733,6 +1744,24 @@ static int32_t combine(Pending &p) {
     const int32_t rc = g_combiner.submit(p, run_pending, err);
     g_last_error = err;
     return rc;
+}
+
+// A workspace's device buffers, cached graphs and pinned chunks, freed (the caller holds its lock). Never throws.
+static void release_buffers(Workspace &w) {
+    Buf *bufs[] = {&w.T,       &w.U,      &w.K0,    &w.K1,     &w.V0,     &w.V1,     &w.SA,     &w.ISA,    &w.L,
+                   &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.carry, &w.samples,
+                   &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
+                   &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
+                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,
+                   &w.sufl,    &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax,   &w.lrow};
+    w.inv_graph.clear();
+    w.fwd_graph.clear();
+    for (Buf *b : bufs) {
+        if (b->p) (void)hipFree(b->p);
+        b->p = nullptr;
+        b->cap = 0;
+    }
+    release_pins(w.pins);
 }
 
 extern "C" {
@@ -2163,20 +2192,7 @@ void kbwt_release(void) {
         }
         release_pins(ln.pins);
     }
-    Buf *bufs[] = {&w.T,       &w.U,      &w.K0,    &w.K1,     &w.V0,     &w.V1,     &w.SA,     &w.ISA,    &w.L,
-                   &w.pos[0],  &w.pos[1], &w.grp[0], &w.grp[1], &w.status, &w.st_sum, &w.carry, &w.samples,
-                   &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
-                   &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
-                   &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,
-                   &w.sufl,    &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax,   &w.lrow};
-    w.inv_graph.clear();
-    w.fwd_graph.clear();
-    for (Buf *b : bufs) {
-        if (b->p) (void)hipFree(b->p);
-        b->p = nullptr;
-        b->cap = 0;
-    }
-    release_pins(w.pins);
+    release_buffers(w);
 }
 
 }  // extern "C"

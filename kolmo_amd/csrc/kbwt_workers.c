@@ -4,6 +4,7 @@
 // (src/core/compress.c:231-256, src/core/decode.c:404-410), and run kolmo_sais_bwt_aux and then
 // kolmo_sais_unbwt_aux on each block from host buffers. Every round trip is compared with its block.
 #include <pthread.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -12,6 +13,17 @@
 #include "kbwt.h"
 
 #define KBWT_WORKERS_API __attribute__((visibility("default")))
+
+// The first failure of the last run: which call, its return code and the library's message.
+static char g_fail[256];
+static int32_t g_fail_set;
+static void note_failure(const char *what, int32_t block, int32_t rc) {
+    if (__atomic_exchange_n(&g_fail_set, 1, __ATOMIC_ACQ_REL)) return;
+    snprintf(g_fail, sizeof g_fail, "%s of block %d: rc %d %s", what, (int)block, (int)rc, rc ? kbwt_last_error() : "");
+}
+
+// The first failure of the last kbwt_workers_run that returned -2, or "".
+KBWT_WORKERS_API const char *kbwt_workers_last_error(void) { return g_fail; }
 
 typedef struct {
     const uint8_t *T;
@@ -38,8 +50,17 @@ static void *worker(void *arg) {
             const int64_t o = (int64_t)k * j->block;
             const int32_t len = (int32_t)(j->n - o < j->block ? j->n - o : j->block);
             const uint8_t *T = j->T + o;
-            if (kolmo_sais_bwt_aux(T, U, A, len, j->rs[k], I, 1) != 0 ||
-                kolmo_sais_unbwt_aux(U, back, A, len, j->rs[k], I, 1) != 0 || memcmp(back, T, (size_t)len) != 0) {
+            int32_t rc = kolmo_sais_bwt_aux(T, U, A, len, j->rs[k], I, 1);
+            if (rc != 0) note_failure("kolmo_sais_bwt_aux", k, rc);
+            if (rc == 0) {
+                rc = kolmo_sais_unbwt_aux(U, back, A, len, j->rs[k], I, 1);
+                if (rc != 0) note_failure("kolmo_sais_unbwt_aux", k, rc);
+                else if (memcmp(back, T, (size_t)len) != 0) {
+                    note_failure("round trip", k, 0);
+                    rc = 1;
+                }
+            }
+            if (rc != 0) {
                 __atomic_store_n(&j->failed, 1, __ATOMIC_RELAXED);
                 break;
             }
@@ -62,6 +83,8 @@ KBWT_WORKERS_API double kbwt_workers_run(const uint8_t *T, int64_t n, int32_t bl
     if (nb > (1 << 20) || n - (nb - 1) * block < 2) return -1.0;
     pthread_t th[256];
     double best = -1.0;
+    g_fail_set = 0;
+    g_fail[0] = 0;
     for (int32_t rep = 0; rep < reps; rep++) {
         Job j = {T, n, block, rs, (int32_t)nb, 0, 0};
         struct timespec t0, t1;
