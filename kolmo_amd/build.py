@@ -15,8 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib")
 ROOT = os.path.dirname(HERE)
 
-HIP_SOURCES = ["kbwt_kernels.hip", "kbwt_rounds.hip", "kbwt_first.hip", "kbwt_router.hip", "kbwt_host.hip",
-               "kbwt_router_host.hip", "kbwt_cpu.cpp"]
+HIP_SOURCES = ["kbwt_kernels.hip", "kbwt_rounds.hip", "kbwt_router.hip", "kbwt_host.hip", "kbwt_router_host.hip",
+               "kbwt_cpu.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared", "-fvisibility=hidden",

@@ -183,7 +183,7 @@ struct Workspace {
     uint32_t dev_epoch_calls = 0;           // inverse calls since the device epoch last started a cycle
     Buf T, U, K0, K1, V0, V1, SA, ISA, L, pos[2], grp[2], status, st_sum, carry, samples, stage, blk, counts, dbase,
         tot, cbase, bad, meta, gh[2], gs[2], lh[2], ls[2], go, sec, sufl, lo, tstart, act, rfeat, bigram, rmeta, eh, es,
-        ecnt, eoff, pair, tmax, lrow, fgt, bh, bs, blo, bK[2], bV[2];
+        ecnt, eoff, pair, tmax, lrow;
     uint32_t *counters = nullptr;           // 16 tile counters (zeroed once); [15]: the device epoch of graphs
     uint32_t *gcnt = nullptr;               // 4 group-list counters
     uint32_t *h_gcnt = nullptr;
@@ -760,19 +760,6 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     }
     memcpy(h_bl, bl.data(), B * sizeof(BlockInfo));
     HIPCHK(hipMemcpyAsync(dm, hm, bytes, hipMemcpyHostToDevice, s));
-    // The first sort: global radix passes over the top Phi digits, then every bucket of rows that share them (at most
-    // FG_CAP rows) sorted in LDS by the low Plo digits, with its groups (k_first_groups). Plo = 3 (the low 32 key bits,
-    // BWT byte included) when the keys have 6 or 7 digits, 2 for 5. KBWT_FIRST_LDS=0: all P0 digits by global passes
-    // and k_regroup.
-    const bool lds_end = env_u32("KBWT_FIRST_LDS", 1, 0, 1) != 0;
-    const uint32_t Plo = lds_end ? (P0 >= 6 ? 3u : 2u) : 0u, Phi = P0 - Plo;
-    const uint32_t hshift = 8 + 8 * Plo, lomask = Plo == 3 ? 0xFFFFFFFFu : 0x00FFFFFFu;
-    const uint32_t nfg = (uint32_t)(((size_t)N + FG_PITCH - 1) / FG_PITCH);
-    if (lds_end) {
-        grow(w.fgt, ((size_t)nfg + 1) * 4);
-        grow(w.bh, (NN / FG_CAP + 2) * 4);
-        grow(w.bs, (NN / FG_CAP + 2) * 4);
-    }
     uint32_t *d_act = nullptr;
     if (feat) {   // router features: the byte histogram now, the active shares per round, the order-1 entropy last
         for (uint32_t b = 0; b < B; b++)
@@ -796,16 +783,6 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         clear_status(w, s);
         w.dev_epoch_calls = 0;
     }
-    // ISA[suffix] = head: partition the (suffix, head) pairs by suffix window inside each block (one keys-only radix
-    // pass), then write window by window instead of scattering 4-byte writes over all of ISA
-    auto isa_from_pairs = [&] {
-        TIMED("isa_part", N,
-              launch_partition_pairs(K[Phi & 1], K[(Phi - 1) & 1], N, tpart, npt, P<uint64_t>(w.status),
-                                     w.counters + 4, MAXP, depoch, s));
-        LAUNCHCHK();
-        TIMED("isa_scatter", N, launch_isa_scatter(K[(Phi - 1) & 1], N, ISA, s));
-        LAUNCHCHK();
-    };
     auto first_sort = [&] {
         launch_epoch_bump(depoch, s);
         // 2. digit histograms of the first keys
@@ -813,17 +790,15 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         TIMED("scan_digits", 256 * P0 * B,
               launch_scan_digits(P<uint32_t>(w.counts), B * P0, P0, d_bl, 1, nullptr, P<uint32_t>(w.dbase), s));
         LAUNCHCHK();
-        // 3. the first sort: Phi passes per block over the top digits (the first builds the keys from T); with
-        // KBWT_FIRST_LDS=0, all P0 digits, the last pass writing the SA
-        for (uint32_t q = 0; q < Phi; q++) {
-            const uint32_t p = Plo + q;
+        // 3. the first sort: P0 stable passes per block over the digits above the BWT byte; the last writes the SA
+        for (uint32_t p = 0; p < P0; p++) {
             SortArgs a{};
             a.cfg = scfg;
-            a.from_t = q == 0;
-            a.kin = q ? K[(q - 1) & 1] : nullptr;
-            a.vin = q ? V[(q - 1) & 1] : nullptr;
-            a.kout = K[q & 1];
-            a.vout = !lds_end && q == Phi - 1 ? SA : V[q & 1];
+            a.from_t = p == 0;
+            a.kin = p ? K[(p - 1) & 1] : nullptr;
+            a.vin = p ? V[(p - 1) & 1] : nullptr;
+            a.kout = K[p & 1];
+            a.vout = p == P0 - 1 ? SA : V[p & 1];
             a.m = N;
             a.shift = 8 * p + 8;
             a.dbase = P<uint32_t>(w.dbase) + p * 256;
@@ -831,39 +806,12 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
             a.tiling = tsort;
             a.status = P<uint64_t>(w.status);
             a.counter = w.counters + 0;
-            a.epoch = q;              // offsets 0..Phi-1 from the device epoch; the partition below takes MAXP
+            a.epoch = p;              // offsets 0..P0-1 from the device epoch; the partition below takes MAXP
             a.depoch = depoch;
             a.T = dT;
             a.codes = d_codes;
-            TIMED(q == 0 ? "sort_pass_T" : "sort_pass", N, sort_pass_checked(a, nst, stile, s));
+            TIMED(p == 0 ? "sort_pass_T" : "sort_pass", N, sort_pass_checked(a, nst, stile, s));
             LAUNCHCHK();
-        }
-        launch_fill_u8(reinterpret_cast<uint8_t *>(w.gcnt), 0, 16, s);
-        if (lds_end) {
-            // 4. buckets sorted by their low digits in LDS, with their groups; the pairs go to the other key buffer,
-            // and the ISA follows once the buckets of more than FG_CAP rows are done (after this sequence)
-            const uint64_t *Kf = K[(Phi - 1) & 1];
-            TIMED("first_tiles", nfg, launch_first_tiles(Kf, N, hshift, blk, d_bl, nfg, P<uint32_t>(w.fgt), s));
-            FirstGroupsArgs f{};
-            f.K = Kf;
-            f.V = V[(Phi - 1) & 1];
-            f.tstart = P<uint32_t>(w.fgt);
-            f.hshift = hshift;
-            f.lomask = lomask;
-            f.blk = blk;
-            f.SA = SA;
-            f.pairs = K[Phi & 1];
-            f.L = L;
-            f.sh = P<uint32_t>(w.gh[0]);
-            f.ss = P<uint32_t>(w.gs[0]);
-            f.lh = P<uint32_t>(w.lh[0]);
-            f.ls = P<uint32_t>(w.ls[0]);
-            f.bh = P<uint32_t>(w.bh);
-            f.bs = P<uint32_t>(w.bs);
-            f.cnt = w.gcnt;
-            TIMED("first_groups", N, launch_first_groups(f, nfg, s));
-            LAUNCHCHK();
-            return;
         }
         // 4. groups after the first sort; the BWT byte of every row is its key's low byte
         RegroupArgs g{};
@@ -885,26 +833,32 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         g.lh = P<uint32_t>(w.lh[0]);
         g.ls = P<uint32_t>(w.ls[0]);
         g.cnt = w.gcnt;
+        launch_fill_u8(reinterpret_cast<uint8_t *>(w.gcnt), 0, 16, s);
         TIMED("regroup", N, launch_regroup(g, s));
         LAUNCHCHK();
-        isa_from_pairs();
+        // ISA[suffix] = head: partition the pairs by suffix window inside each block (one keys-only radix pass),
+        // then write window by window instead of scattering 4-byte writes over all of ISA
+        TIMED("isa_part", N,
+              launch_partition_pairs(K[P0 & 1], K[(P0 - 1) & 1], N, tpart, npt, P<uint64_t>(w.status),
+                                     w.counters + 4, MAXP, depoch, s));
+        LAUNCHCHK();
+        TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, ISA, s));
+        LAUNCHCHK();
     };
     if (feat) {   // the router's path: its per-round counts are set up around this sequence, so no graph
         first_sort();
     } else {
         const std::vector<uint64_t> key = {
-            B, N, P0, Plo, (uint64_t)scfg, nht, nst, npt, (uintptr_t)dT, (uintptr_t)dm, (uintptr_t)w.counts.p,
+            B, N, P0, (uint64_t)scfg, nht, nst, npt, (uintptr_t)dT, (uintptr_t)dm, (uintptr_t)w.counts.p,
             (uintptr_t)w.dbase.p, (uintptr_t)w.K0.p, (uintptr_t)w.K1.p, (uintptr_t)w.V0.p, (uintptr_t)w.V1.p,
             (uintptr_t)w.SA.p, (uintptr_t)w.ISA.p, (uintptr_t)w.L.p, (uintptr_t)w.carry.p, (uintptr_t)w.gh[0].p,
             (uintptr_t)w.gs[0].p, (uintptr_t)w.lh[0].p, (uintptr_t)w.ls[0].p, (uintptr_t)w.status.p,
-            (uintptr_t)w.counters, (uintptr_t)w.gcnt, (uintptr_t)blk, (uintptr_t)w.fgt.p, (uintptr_t)w.bh.p,
-            (uintptr_t)w.bs.p};
+            (uintptr_t)w.counters, (uintptr_t)w.gcnt, (uintptr_t)blk};
         run_graph(w.fwd_graph, key, s,
-                  {"hist_T", "scan_digits", "sort_pass_T", "sort_pass", "first_tiles", "first_groups", "regroup",
-                   "isa_part", "isa_scatter"},
+                  {"hist_T", "scan_digits", "sort_pass_T", "sort_pass", "regroup", "isa_part", "isa_scatter"},
                   first_sort);
     }
-    st.passes[0] = (int32_t)Phi;
+    st.passes[0] = (int32_t)P0;
     st.active[0] = N;
 
     // 5. doubling rounds over the groups of two or more (SA ranges). Small groups sort inside one workgroup each
@@ -951,103 +905,8 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     auto zero_counts = [&] { launch_fill_u8(reinterpret_cast<uint8_t *>(gcnt), 0, 16, s); };
     int cur = 0, round = 0;
     uint32_t Gs = 0, Gl = 0;
-    read_lists(0, Gs, Gl);   // the lists the first sort wrote
+    read_lists(0, Gs, Gl);   // the lists the first regroup wrote
     const Tiling flat{nullptr, d_bl, 1};
-    if (lds_end) {
-        // Buckets of more than FG_CAP rows (their count and rows came back with the list counters): one element list
-        // in increasing row order, keyed (bucket << 32 | low 32 key bits), sorted by the radix passes over the digits
-        // that vary above the BWT byte, and split into groups by k_regroup, which writes their SA, L and pairs rows.
-        // Then the ISA from all pairs.
-        const uint32_t nbig = w.h_gcnt[2], mbig = w.h_gcnt[3];
-        if (nbig) {
-            std::vector<uint32_t> hb(2 * (size_t)nbig), hbl(2 * (size_t)nbig);
-            HIPCHK(hipMemcpyAsync(hb.data(), w.bh.p, (size_t)nbig * 4, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipMemcpyAsync(hb.data() + nbig, w.bs.p, (size_t)nbig * 4, hipMemcpyDeviceToHost, s));
-            SYNCCHK(s);
-            std::vector<uint32_t> ord(nbig);
-            for (uint32_t g = 0; g < nbig; g++) ord[g] = g;
-            std::sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) { return hb[x] < hb[y]; });
-            uint64_t m = 0;
-            for (uint32_t k = 0; k < nbig; k++) {
-                hbl[k] = hb[ord[k]];
-                hbl[nbig + k] = (uint32_t)m;
-                m += hb[nbig + ord[k]];
-            }
-            if (m != mbig || m > N) fail(hipErrorUnknown, "first sort: bucket rows do not add up");
-            grow(w.blo, (size_t)2 * nbig * 4);
-            for (int i = 0; i < 2; i++) {
-                grow(w.bK[i], (size_t)mbig * 8);
-                grow(w.bV[i], (size_t)mbig * 4);
-            }
-            HIPCHK(hipMemcpyAsync(w.blo.p, hbl.data(), (size_t)2 * nbig * 4, hipMemcpyHostToDevice, s));
-            HIPCHK(hipMemcpyAsync(w.orand, w.h_orand_init, 16, hipMemcpyHostToDevice, s));
-            uint64_t *bK[2] = {P<uint64_t>(w.bK[0]), P<uint64_t>(w.bK[1])};
-            uint32_t *bV[2] = {P<uint32_t>(w.bV[0]), P<uint32_t>(w.bV[1])};
-            TIMED("big_elems", mbig,
-                  launch_big_elems(P<uint32_t>(w.blo), P<uint32_t>(w.blo) + nbig, nbig, mbig, K[(Phi - 1) & 1],
-                                   V[(Phi - 1) & 1], lomask, P<uint32_t>(w.pos[0]), bK[0], bV[0], w.orand, s));
-            LAUNCHCHK();
-            HIPCHK(hipMemcpyAsync(w.h_orand, w.orand, 16, hipMemcpyDeviceToHost, s));
-            SYNCCHK(s);
-            Shifts bsh{};
-            const uint64_t vary = w.h_orand[0] ^ w.h_orand[1];
-            for (uint32_t d = 1; d < 8; d++)
-                if ((vary >> (8 * d)) & 0xFFull) bsh.s[bsh.P++] = 8 * d;
-            int src = 0;
-            if (bsh.P) {
-                const uint32_t g2 = std::max(1u, std::min<uint32_t>(HIST_GRID, (mbig + 8191) / 8192));
-                TIMED("hist_keys", mbig, launch_hist_keys(bK[0], mbig, bsh, P<uint32_t>(w.counts), g2, s));
-                TIMED("scan_digits", 256 * bsh.P,
-                      launch_scan_digits(P<uint32_t>(w.counts), bsh.P, MAXP, d_bl, 0, nullptr, P<uint32_t>(w.dbase),
-                                         s));
-                LAUNCHCHK();
-                const uint32_t ntl = (mbig + stile - 1) / stile;
-                for (uint32_t p = 0; p < bsh.P; p++) {
-                    SortArgs a{};
-                    a.cfg = scfg;
-                    a.from_t = false;
-                    a.kin = bK[src];
-                    a.vin = bV[src];
-                    a.kout = bK[src ^ 1];
-                    a.vout = bV[src ^ 1];
-                    a.m = mbig;
-                    a.shift = bsh.s[p];
-                    a.dbase = P<uint32_t>(w.dbase) + p * 256;
-                    a.dstride = 0;
-                    a.tiling = flat;
-                    a.status = P<uint64_t>(w.status);
-                    a.counter = w.counters + 0;
-                    a.epoch = next_epoch(w, s);
-                    TIMED("sort_pass", mbig, sort_pass_checked(a, ntl, stile, s));
-                    LAUNCHCHK();
-                    src ^= 1;
-                }
-            }
-            RegroupArgs g{};
-            g.K = bK[src];
-            g.V = bV[src];
-            g.pos = P<uint32_t>(w.pos[0]);
-            g.m = mbig;
-            g.kshift = 8;
-            g.blk = nullptr;
-            g.blocks = d_bl;
-            g.SA = SA;
-            g.ISA = ISA;
-            g.pairs = K[Phi & 1];
-            g.L = L;
-            g.T = nullptr;
-            g.carry = P<uint32_t>(w.carry);
-            g.sh = P<uint32_t>(w.gh[0]);
-            g.ss = P<uint32_t>(w.gs[0]);
-            g.lh = P<uint32_t>(w.lh[0]);
-            g.ls = P<uint32_t>(w.ls[0]);
-            g.cnt = gcnt;
-            TIMED("regroup", mbig, launch_regroup(g, s));
-            LAUNCHCHK();
-        }
-        isa_from_pairs();
-        if (nbig) read_lists(0, Gs, Gl);   // with the big buckets' groups
-    }
     while (Gs + Gl > 0) {
         round++;
         if (round >= KBWT_MAX_ROUNDS) fail(hipErrorUnknown, "doubling did not converge");
@@ -1894,8 +1753,7 @@ static void release_buffers(Workspace &w) {
                    &w.stage,   &w.blk,    &w.counts, &w.dbase,  &w.tot,    &w.cbase,  &w.bad,    &w.meta,
                    &w.gh[0],   &w.gh[1],  &w.gs[0],  &w.gs[1],  &w.lh[0],  &w.lh[1],  &w.ls[0],  &w.ls[1],
                    &w.go,      &w.sec,    &w.lo,     &w.tstart, &w.act,    &w.rfeat,  &w.bigram, &w.rmeta,
-                   &w.sufl,    &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax,   &w.lrow,
-                   &w.fgt,     &w.bh,     &w.bs,     &w.blo,    &w.bK[0],  &w.bK[1],  &w.bV[0],  &w.bV[1]};
+                   &w.sufl,    &w.eh,     &w.es,     &w.ecnt,   &w.eoff,   &w.pair,   &w.tmax,   &w.lrow};
     w.inv_graph.clear();
     w.fwd_graph.clear();
     for (Buf *b : bufs) {

@@ -110,31 +110,6 @@ struct SegRoundArgs {
     uint32_t rank_max;            // tiles whose groups have at most rank_max rows rank them directly
 };
 
-// The end of the first sort (kbwt_first.hip): buckets of rows that share the top digits of their first keys, sorted
-// in LDS by the low digits. Tile t owns the buckets that start in [t * FG_PITCH, (t + 1) * FG_PITCH); buckets of more
-// than FG_CAP rows take the global path.
-constexpr uint32_t FG_PITCH = 2048, FG_CAP = 2048;
-
-struct FirstGroupsArgs {
-    const uint64_t *K;            // keys sorted by their top digits (K >> hshift) within each block
-    const uint32_t *V;            // their suffixes
-    const uint32_t *tstart;       // first row of every tile's range (k_first_tiles), and N at the end
-    uint32_t hshift, lomask;      // top digits: K >> hshift; low digits and BWT byte: K & lomask
-    const uint16_t *blk;          // block of every packed index, or nullptr for one block
-    uint32_t *SA;
-    uint64_t *pairs;              // (suffix << 32 | head) per row
-    uint8_t *L;
-    uint32_t *sh, *ss, *lh, *ls;  // the doubling rounds' first small and large group lists
-    uint32_t *bh, *bs;            // buckets left to the global path: first row, rows
-    uint32_t *cnt;                // cnt[0] small groups, cnt[1] large groups, cnt[2] big buckets, cnt[3] their rows
-};
-void launch_first_tiles(const uint64_t *K, uint32_t N, uint32_t hshift, const uint16_t *blk, const BlockInfo *blocks,
-                        uint32_t ntiles, uint32_t *tstart, hipStream_t s);
-void launch_first_groups(const FirstGroupsArgs &a, uint32_t ntiles, hipStream_t s);
-void launch_big_elems(const uint32_t *bh, const uint32_t *lo, uint32_t G, uint32_t m, const uint64_t *K,
-                      const uint32_t *V, uint32_t lomask, uint32_t *pos, uint64_t *K2, uint32_t *V2,
-                      unsigned long long *orand, hipStream_t s);
-
 void launch_scan_groups(const uint32_t *gs, uint32_t G, uint32_t *go, uint32_t *tot, uint32_t *tstart,
                         uint64_t *status, uint32_t *counter, uint32_t epoch, hipStream_t s);
 void launch_compact_emits(const uint32_t *eh, const uint32_t *es, const uint32_t *ecnt, const uint32_t *eoff,

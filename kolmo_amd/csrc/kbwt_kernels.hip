@@ -489,7 +489,7 @@ __global__ __launch_bounds__(256) void k_regroup_carry(const uint64_t *__restric
 
 // Sorted keys K[0..m) with values V (suffixes). Row a goes to SA position pos[a] (pos == nullptr: a). Writes
 // SA[pos[a]] = V[a] (when SA != nullptr), ISA[V[a]] = head of its group (with pairs, the pair (V[a] << 32 | head)
-// goes to pairs[pos[a]] instead, for k_isa_scatter after a partition by suffix), the BWT byte of row pos[a] into L (the
+// goes to pairs[a] instead, for k_isa_scatter after a partition by suffix), the BWT byte of row pos[a] into L (the
 // key's low byte when T == nullptr, else T[V[a]-1]; the byte of a block's first suffix is never used), and appends
 // every group of two or more rows as (head, size) to the small list (sh, ss; size <= SEG_GROUP) or the large list
 // (lh, ls), one atomic per list and tile on cnt[0], cnt[1]; the lists' order is free. carry[tile]: the head of the
@@ -636,12 +636,11 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
         else if (es[k]) { sh[at_s] = eh[k]; ss[at_s] = es[k]; at_s++; }
     }
     if (!pairs) return;
-    // stores of the pairs in list order (the barrier above ordered the pair writes before these reads): coalesced at
-    // list index, or at the row pos[a]
+    // coalesced stores of the pairs in row order (the barrier above ordered the pair writes before these reads)
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const uint32_t j = k * BLOCK + tid;
-        if (j < cnt) pairs[pos ? pos[t0 + j] : t0 + j] = s_k[at(j)];
+        if (j < cnt) pairs[t0 + j] = s_k[at(j)];
     }
 }
 
@@ -1120,7 +1119,6 @@ __global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const u
 }
 
 // ------------------------------------------------------------------ launch wrappers
-#ifndef KBWT_CPU_EMU   // the launch wrappers need hipcc; a CPU build of the kernels above leaves them out
 
 // The text histograms run four workgroups per CU (their LDS allows it): each one's LDS atomics wait long, and more
 // waves per SIMD hide that (k_hist_T: 0.131 -> 0.103 ms on a 32 MiB block against 512 workgroups).
@@ -1272,7 +1270,5 @@ void launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s) {
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s) {
     if (n) k_fill_u8<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, v, n);
 }
-
-#endif
 
 }  // namespace kbwt
