@@ -1,0 +1,77 @@
+"""Shapes of the first sort's key space (GPU): runs of one byte inside random bytes (a run of R bytes gives about R rows
+whose first keys agree, one group the doubling rounds split), whole blocks of one or two byte values, keys of five
+8-bit digits (20- and 30-symbol alphabets, 5-bit codes), and such blocks mixed in one batch, where every block keeps its
+own code width. Bar: bit-exact against the oracle, and every block round-trips.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import kolmo_amd as K
+import refs
+
+pytestmark = pytest.mark.gpu
+CAP = 2048   # run lengths around a workgroup's tile of 2048 rows
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def check(T, tag):
+    r = 1 << K.codec18_rlog(T.size)
+    U, I = K.bwt_aux(T, r)
+    eU, eI = refs.oracle_bwt(T, r)
+    assert np.array_equal(U, eU), f"{tag}: BWT bytes differ"
+    assert np.array_equal(I, eI), f"{tag}: sampled indexes differ"
+    assert np.array_equal(K.unbwt_aux(U, r, I), T), f"{tag}: inverse"
+
+
+def with_run(n, run, seed, at=None, byte=97):
+    rng = np.random.default_rng(seed)
+    T = rng.integers(0, 256, n, dtype=np.uint8)
+    at = n // 3 if at is None else at
+    T[at:at + run] = byte
+    return T
+
+
+def small_alphabet(n, sigma, seed):
+    rng = np.random.default_rng(seed)
+    p = rng.dirichlet(np.full(sigma, 0.5))
+    return (rng.choice(sigma, n, p=p) + 65).astype(np.uint8)
+
+
+def cases():
+    yield "one byte", np.full(100000, 7, np.uint8)
+    yield "two bytes", (np.random.default_rng(3).integers(0, 2, 300000) + 48).astype(np.uint8)
+    for run in (CAP + 2, CAP + 3, CAP + 4, CAP + 5, 2 * CAP + 10, 5 * CAP):
+        yield f"run {run}", with_run(300000, run, run)
+    yield "runs at tile edges", np.concatenate([with_run(2 * CAP + 7, CAP + 3, 11, at=5), with_run(CAP * 3, CAP + 4, 12,
+                                                                                                   at=CAP - 9)])
+    yield "20 symbols", small_alphabet(600000, 20, 4)
+    yield "30 symbols", small_alphabet(300000, 30, 5)
+    for kind, n in [("repeats", 1 << 20), ("dna", 700000), ("mixed", 700000), ("logs", 300000), ("csvcols", 300000),
+                    ("random", 200000)]:
+        yield kind, K.generate(kind, 61, n)
+
+
+def test_gpu_first_sort_shapes(gpu):
+    for tag, T in cases():
+        say("first sort", tag, T.size)
+        check(T, tag)
+
+
+def test_gpu_first_sort_batch(gpu):
+    """Blocks of different key widths and run shapes in one batch (groups never cross a block)."""
+    blocks = [small_alphabet(200000, 20, 8), with_run(150000, CAP + 4, 9), np.full(5000, 3, np.uint8),
+              K.generate("mixed", 62, 300000), with_run(4100, CAP + 3, 10, at=0), K.generate("dna", 63, 90000),
+              np.frombuffer(b"ab", np.uint8).copy(), small_alphabet(70000, 30, 12)]
+    rs = [1 << K.codec18_rlog(b.size) for b in blocks]
+    say("first sort batch", [b.size for b in blocks])
+    Us, Is = K.bwt_aux_batch(blocks, rs)
+    for b, (T, r, U, I) in enumerate(zip(blocks, rs, Us, Is)):
+        eU, eI = refs.oracle_bwt(T, r)
+        assert np.array_equal(U, eU) and np.array_equal(I, eI), f"block {b}"
+        assert np.array_equal(K.unbwt_aux(U, r, I), T), f"block {b} inverse"
