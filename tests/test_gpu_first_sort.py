@@ -1,7 +1,8 @@
 """Shapes of the first sort's key space (GPU): runs of one byte inside random bytes (a run of R bytes gives about R rows
 whose first keys agree, one group the doubling rounds split), whole blocks of one or two byte values, keys of five
 8-bit digits (20- and 30-symbol alphabets, 5-bit codes), and such blocks mixed in one batch, where every block keeps its
-own code width. Bar: bit-exact against the oracle, and every block round-trips.
+own code width; key widths of five, six and seven 8-bit digits. Bar: bit-exact against the oracle, and every block
+round-trips.
 """
 import sys
 
@@ -51,6 +52,7 @@ def cases():
     yield "runs at tile edges", np.concatenate([with_run(2 * CAP + 7, CAP + 3, 11, at=5), with_run(CAP * 3, CAP + 4, 12,
                                                                                                    at=CAP - 9)])
     yield "20 symbols", small_alphabet(600000, 20, 4)
+    yield "64 symbols", small_alphabet(500000, 64, 6)
     yield "30 symbols", small_alphabet(300000, 30, 5)
     for kind, n in [("repeats", 1 << 20), ("dna", 700000), ("mixed", 700000), ("logs", 300000), ("csvcols", 300000),
                     ("random", 200000)]:
