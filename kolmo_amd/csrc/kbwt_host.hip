@@ -295,8 +295,10 @@ Workspace &workspace() {
             const uint32_t e0 = EPOCH_HALF;
             HIPCHK(hipMemcpy(w.counters + 15, &e0, 4, hipMemcpyHostToDevice));
         }
-        HIPCHK(hipEventCreate(&w.ev0));
-        HIPCHK(hipEventCreate(&w.ev1));
+        // timing only, read after the stream is synchronised: no system-scope fence (a fence writes back and
+        // invalidates the caches at every record, which slows the work around it)
+        HIPCHK(hipEventCreateWithFlags(&w.ev0, hipEventDisableSystemFence));
+        HIPCHK(hipEventCreateWithFlags(&w.ev1, hipEventDisableSystemFence));
         w.ready = true;
     }
     return w;
@@ -514,9 +516,9 @@ template <class F> void timed(Workspace &w, hipStream_t s, const char *name, uin
         w.spare.pop_back();
         b = w.spare.back();
         w.spare.pop_back();
-    } else {
-        HIPCHK(hipEventCreate(&a));
-        HIPCHK(hipEventCreate(&b));
+    } else {   // timing only (prof_collect reads them after a synchronisation): no system-scope fence
+        HIPCHK(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
+        HIPCHK(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
     }
     HIPCHK(hipEventRecord(a, s));
     fn();
