@@ -707,10 +707,12 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     grow(w.dbase, (size_t)B * MAXP * 256 * 4);
     if (B > 1) grow(w.blk, NN * 2);
 
-    // metadata layout: blocks | hist tiles | sort tiles | partition tiles | codes; the byte counts come back behind it
+    // metadata layout: blocks | hist tiles | sort tiles | partition tiles | codes | m-gram units; the byte counts come
+    // back behind it
     const size_t o_ht = align16(B * sizeof(BlockInfo)), o_st = o_ht + (B + 1) * 4, o_pt = o_st + (B + 1) * 4;
     const size_t o_codes = align16(o_pt + (B + 1) * 4);
-    const size_t bytes = o_codes + (size_t)B * 256;
+    const size_t o_gu = align16(o_codes + (size_t)B * 256);
+    const size_t bytes = o_gu + (B + 1) * 4;
     uint8_t *hm = (uint8_t *)host_staging(w, bytes + (size_t)B * 256 * 4);
     grow(w.meta, bytes);
     uint8_t *dm = P<uint8_t>(w.meta);
@@ -760,6 +762,17 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
             st.h0 = (int32_t)bl[b].h0;
         }
     }
+    // digit histograms of the first keys: m-gram units (8-bit codes: one unit, from the byte counts) or, for 5-bit
+    // codes, k_hist_T's tiles
+    bool plain_hist = false;
+    const uint32_t ngu = fill_tiles((uint32_t *)(hm + o_gu), B, [&](uint32_t b) {
+        if (!gram_block(bl[b].w)) {
+            plain_hist = true;
+            return 0u;
+        }
+        return bl[b].w == 8 ? 1u : (bl[b].n + GRAM_UNIT - 1) / GRAM_UNIT;
+    });
+    const Tiling tgram{(const uint32_t *)(dm + o_gu), d_bl, B};
     memcpy(h_bl, bl.data(), B * sizeof(BlockInfo));
     HIPCHK(hipMemcpyAsync(dm, hm, bytes, hipMemcpyHostToDevice, s));
     uint32_t *d_act = nullptr;
@@ -788,7 +801,10 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     auto first_sort = [&] {
         launch_epoch_bump(depoch, s);
         // 2. digit histograms of the first keys
-        TIMED("hist_T", N, launch_hist_T(dT, thist, nht, d_codes, P0, P<uint32_t>(w.counts), s));
+        TIMED("hist_T", N, {
+            if (plain_hist) launch_hist_T(dT, thist, nht, d_codes, P0, P<uint32_t>(w.counts), s);
+            launch_hist_gram(dT, tgram, ngu, d_codes, P<uint32_t>(w.tot), P0, P<uint32_t>(w.counts), s);
+        });
         TIMED("scan_digits", 256 * P0 * B,
               launch_scan_digits(P<uint32_t>(w.counts), B * P0, P0, d_bl, 1, nullptr, P<uint32_t>(w.dbase), s));
         LAUNCHCHK();
@@ -851,7 +867,8 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
         first_sort();
     } else {
         const std::vector<uint64_t> key = {
-            B, N, P0, (uint64_t)scfg, nht, nst, npt, (uintptr_t)dT, (uintptr_t)dm, (uintptr_t)w.counts.p,
+            B, N, P0, (uint64_t)scfg, nht, nst, npt, ngu, (uint64_t)plain_hist, (uintptr_t)w.tot.p, (uintptr_t)dT,
+            (uintptr_t)dm, (uintptr_t)w.counts.p,
             (uintptr_t)w.dbase.p, (uintptr_t)w.K0.p, (uintptr_t)w.K1.p, (uintptr_t)w.V0.p, (uintptr_t)w.V1.p,
             (uintptr_t)w.SA.p, (uintptr_t)w.ISA.p, (uintptr_t)w.L.p, (uintptr_t)w.carry.p, (uintptr_t)w.gh[0].p,
             (uintptr_t)w.gs[0].p, (uintptr_t)w.lh[0].p, (uintptr_t)w.ls[0].p, (uintptr_t)w.status.p,

@@ -13,6 +13,16 @@ constexpr int MAXP = 8;                      // 8-bit digits of a 64-bit key
 constexpr int RG_BLOCK = 256, RG_ITEMS = 8, RG_TILE = RG_BLOCK * RG_ITEMS;
 constexpr int LF_BLOCK = 256, LF_ITEMS = 32, LF_TILE = LF_BLOCK * LF_ITEMS;
 constexpr int HIST_TILE = 4096;              // bytes per tile of the histogram kernels
+constexpr uint32_t GRAM_UNIT = 1u << 17;     // bytes per workgroup of the m-gram histograms
+constexpr uint32_t GRAM_MAXBITS = 14;        // widest m-gram (bins of one LDS table)
+
+// An 8-bit digit of a first key, counted from the key's bottom, overlaps at most gram_len(w) consecutive codes of w
+// bits. Blocks with gram_len(w) * w <= GRAM_MAXBITS (every width but 5 bits) take the m-gram digit histograms.
+__host__ __device__ inline uint32_t gram_len(uint32_t w) {
+    constexpr uint8_t len[9] = {8, 8, 4, 4, 2, 3, 2, 2, 1};
+    return w <= 8 ? len[w] : 8;
+}
+__host__ __device__ inline bool gram_block(uint32_t w) { return gram_len(w) * w <= GRAM_MAXBITS; }
 constexpr int HIST_GRID = 512;               // workgroups of k_hist_keys
 constexpr int WALK_TILE = 256;               // splitters per workgroup of the inverse walks
 constexpr uint32_t MIN_TILE = 2048;          // smallest tile of any tiled kernel (sizes the lookback arrays)
@@ -123,6 +133,8 @@ void launch_scan_digits(uint32_t *counts, uint32_t nsets, uint32_t sets_per_bloc
                         int base_mode, uint32_t *tot, uint32_t *excl, hipStream_t s);
 void launch_hist_T(const uint8_t *T, Tiling tl, uint32_t ntiles, const uint8_t *codes, uint32_t P, uint32_t *counts,
                    hipStream_t s);
+void launch_hist_gram(const uint8_t *T, Tiling units, uint32_t nunits, const uint8_t *codes, const uint32_t *tot,
+                      uint32_t P, uint32_t *counts, hipStream_t s);
 void launch_hist_keys(const uint64_t *K, uint32_t m, const Shifts &sh, uint32_t *counts, uint32_t grid,
                       hipStream_t s);
 uint32_t sort_tile(int cfg);

@@ -216,6 +216,7 @@ __global__ __launch_bounds__(256) void k_hist_T(const uint8_t *__restrict__ T, T
     };
     for (uint32_t t = t0; t < t1; t++) {
         TileRange r = tile_range(tl, t, HIST_TILE, 0);
+        if (gram_block(tl.blocks[r.b].w)) continue;   // k_hist_gram's block
         __syncthreads();
         if ((int)r.b != cur || since == (uint32_t)HT_FLUSH) {
             if (cur >= 0) flush((uint32_t)cur);
@@ -250,6 +251,221 @@ __global__ __launch_bounds__(256) void k_hist_T(const uint8_t *__restrict__ T, T
     }
     __syncthreads();
     if (cur >= 0) flush((uint32_t)cur);
+}
+
+// m-gram digit histograms. Digit p of the first key of suffix i is bits [8p, 8p + 8) of its h0 codes (the code of
+// T[i + h0 - 1] lowest, so code c is the symbol at i + h0 - 1 - c). It overlaps at most m = gram_len(w) consecutive
+// codes, from lo = floor(8p / w) up, so it is a bit field of the m-gram G(j) = x_j .. x_{j+m-1} (x the codes, zero
+// outside the block, the first one highest) at j = i + o, o = h0 - lo - m, with the gram's codes before position i
+// masked off. A block's digit histograms are therefore the histogram M of G(j) over j in [0, n), each gram adding its
+// count to the bin it gives in every digit, corrected at the block's ends: the grams G(i + o) that start outside
+// [0, n) are added and the grams in [0, n) that belong to no suffix removed (at most |o| <= 56 of each per digit).
+// That is one LDS count per position instead of one per digit.
+__device__ __forceinline__ uint32_t gram_digit(uint32_t g, uint32_t p, uint32_t w, uint32_t h0, uint32_t m) {
+    if (8 * p >= h0 * w) return 0;   // above the key: always zero
+    const uint32_t lo = 8 * p / w, s = 8 * p - w * lo, kb = w * (h0 - lo);   // kb: gram bits inside the key
+    return ((kb < m * w ? g & ((1u << kb) - 1u) : g) >> s) & 255u;
+}
+
+constexpr int GRAM_BLOCK = 1024, GRAM_CHUNK = GRAM_BLOCK * 16;
+static_assert(GRAM_UNIT % GRAM_CHUNK == 0, "whole chunks per unit");
+
+// Bytes [p, p + 16) of a block ending at lim and how many of them lie inside it; one aligned vector load when all do
+// (the caller keeps p 16-byte aligned then).
+struct Raw16 {
+    uint4 v;
+    uint32_t nv;
+};
+__device__ __forceinline__ Raw16 load_raw16(const uint8_t *__restrict__ T, size_t p, size_t lim) {
+    Raw16 r;
+    r.nv = p >= lim ? 0u : (lim - p >= 16 ? 16u : (uint32_t)(lim - p));
+    if (r.nv == 16) {
+        r.v = *reinterpret_cast<const uint4 *>(T + p);
+    } else {
+        uint32_t wd[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++)
+            if (k < r.nv) wd[k >> 2] |= (uint32_t)T[p + k] << (8 * (k & 3));
+        r.v = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+    }
+    return r;
+}
+// The codes of a Raw16 (0 past the block), packed the same way.
+__device__ __forceinline__ uint4 codes16(const Raw16 &r, const uint8_t *s_code) {
+    const uint32_t wd[4] = {r.v.x, r.v.y, r.v.z, r.v.w};
+    uint32_t o[4];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; q++) {
+        uint32_t x = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++)
+            if (4 * q + k < r.nv) x |= (uint32_t)s_code[(wd[q] >> (8 * k)) & 255u] << (8 * k);
+        o[q] = x;
+    }
+    return make_uint4(o[0], o[1], o[2], o[3]);
+}
+// Count the M-grams starting at chunk positions j0 .. j0 + 15 below cnt: the 16 + 8 codes from j0 in two LDS loads,
+// then 16 independent LDS atomics.
+template <uint32_t M>
+__device__ __forceinline__ void count_grams(const uint8_t *s_c, uint32_t j0, uint32_t cnt, uint32_t w, uint32_t gmask,
+                                            uint32_t *h) {
+    const uint4 a = *reinterpret_cast<const uint4 *>(s_c + j0);
+    const uint2 b = *reinterpret_cast<const uint2 *>(s_c + j0 + 16);
+    const uint32_t wd[6] = {a.x, a.y, a.z, a.w, b.x, b.y};
+    auto code = [&](uint32_t k) { return (wd[k >> 2] >> (8 * (k & 3))) & 255u; };
+    uint32_t g = 0;
+#pragma unroll
+    for (uint32_t k = 0; k + 1 < M; k++) g = (g << w) | code(k);
+    if (j0 + 16 <= cnt) {
+#pragma unroll
+        for (uint32_t i = 0; i < 16; i++) {
+            g = ((g << w) | code(i + M - 1)) & gmask;
+            atomicAdd(&h[g], 1u);
+        }
+    } else {
+#pragma unroll
+        for (uint32_t i = 0; i < 16; i++)
+            if (j0 + i < cnt) {
+                g = ((g << w) | code(i + M - 1)) & gmask;
+                atomicAdd(&h[g], 1u);
+            }
+    }
+}
+
+// One workgroup per GRAM_UNIT of a block with gram_block(w): M of the unit in LDS, mapped to the P digit histograms in
+// LDS, added to counts[(b*P + p)*256 + d]. The workgroup of the block's first unit adds the end corrections. 8-bit
+// codes take one unit per block and no pass over T: their 1-grams are the byte histogram tot. The unit goes through
+// LDS in chunks of 16 codes per thread; when the unit starts 16-byte aligned, each thread translates its 16 bytes in
+// registers, stores them with one vector write and loads the next chunk's bytes while the current one is counted.
+__global__ __launch_bounds__(GRAM_BLOCK) void k_hist_gram(const uint8_t *__restrict__ T, Tiling units,
+                                                          const uint8_t *__restrict__ codes,
+                                                          const uint32_t *__restrict__ tot, uint32_t P,
+                                                          uint32_t *__restrict__ counts) {
+    __shared__ uint32_t h[1u << GRAM_MAXBITS];
+    __shared__ uint32_t dh[MAXP][256];
+    __shared__ uint8_t s_code[256];
+    __shared__ __align__(16) uint8_t s_c[GRAM_CHUNK + 16];   // a chunk's codes and the next few
+    __shared__ uint8_t s_end[2][64];                         // codes of the block's first and last 64 positions
+    const uint32_t tid = threadIdx.x;
+    const TileRange r = tile_range(units, blockIdx.x, GRAM_UNIT, 0);
+    const BlockInfo bi = units.blocks[r.b];
+    const uint32_t w = bi.w, h0 = bi.h0, n = bi.n, m = gram_len(w), nb = 1u << (m * w), gmask = nb - 1u;
+    for (uint32_t i = tid; i < nb; i += GRAM_BLOCK) h[i] = 0;
+    for (uint32_t i = tid; i < MAXP * 256; i += GRAM_BLOCK) (&dh[0][0])[i] = 0;
+    if (tid < 256) s_code[tid] = codes[(size_t)r.b * 256 + tid];
+    __syncthreads();
+    if (r.tb == 0 && tid < 128) {
+        const uint32_t e = tid >> 6, k = tid & 63u;
+        const int64_t q = e ? (int64_t)n - 64 + k : (int64_t)k;
+        s_end[e][k] = q >= 0 && q < (int64_t)n ? s_code[T[(size_t)bi.off + (size_t)q]] : 0;
+    }
+    if (w == 8) {   // distinct bytes have distinct codes
+        if (tid < 256) {
+            const uint32_t c = tot[(size_t)r.b * 256 + tid];
+            if (c) h[s_code[tid]] = c;
+        }
+    } else {
+        const uint32_t nch = (uint32_t)((r.end - r.beg + GRAM_CHUNK - 1) / GRAM_CHUNK);
+        const bool vec = ((reinterpret_cast<uintptr_t>(T) + r.beg) & 15u) == 0;   // then every chunk starts aligned
+        Raw16 v{};                  // this thread's 16 bytes of the chunk to come
+        uint32_t la = 0, la_in = 0;   // threads 0-7: one byte past that chunk, and whether it lies in the block
+        auto fetch = [&](size_t c0) {
+            v = load_raw16(T, c0 + 16 * tid, r.bend);
+            if (tid < 8) {
+                const size_t q = c0 + GRAM_CHUNK + tid;
+                la_in = q < r.bend;
+                la = la_in ? T[q] : 0u;
+            }
+        };
+        if (vec) fetch(r.beg);
+        for (uint32_t k = 0; k < nch; k++) {
+            const size_t c0 = r.beg + (size_t)k * GRAM_CHUNK;
+            const uint32_t cnt = (uint32_t)(r.end - c0 < (size_t)GRAM_CHUNK ? r.end - c0 : (size_t)GRAM_CHUNK);
+            if (vec) {
+                *reinterpret_cast<uint4 *>(s_c + 16 * tid) = codes16(v, s_code);
+                if (tid < 8) s_c[GRAM_CHUNK + tid] = la_in ? s_code[la] : 0;
+                __syncthreads();
+                if (k + 1 < nch) fetch(c0 + GRAM_CHUNK);
+            } else {
+                stage_bytes<GRAM_BLOCK>(T, c0, cnt + m - 1, r.bend,
+                                        [&](uint32_t j, uint8_t t) { s_c[j] = c0 + j < r.bend ? s_code[t] : 0; });
+                __syncthreads();
+            }
+            const uint32_t j0 = 16 * tid;
+            if (j0 < cnt) {
+                if (m == 2) count_grams<2>(s_c, j0, cnt, w, gmask, h);
+                else if (m == 4) count_grams<4>(s_c, j0, cnt, w, gmask, h);
+                else count_grams<8>(s_c, j0, cnt, w, gmask, h);
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    // The unit's digit histograms. Digit p is bits [s, s + wd) of a gram (s = 8p - w*lo <= 6, wd = min(8, kb - s)), so
+    // hist_p[d] is the sum of the entries y = d mod 2^wd of L_s, the table with its low s bits summed out. L_0 = h and
+    // L_{s+1}[y] = L_s[2y] + L_s[2y+1] are built in place; every bin is one sum over consecutive words (no atomics).
+    // A digit above the key is 0 for every position.
+    uint32_t smax = 0;
+    for (uint32_t p = 0; p < P; p++)
+        if (8 * p < h0 * w) smax = max(smax, 8 * p - w * (8 * p / w));
+    if (tid < P && 8 * tid >= h0 * w) dh[tid][0] = w == 8 ? n : (uint32_t)(r.end - r.beg);
+    uint32_t len = nb;
+    for (uint32_t s = 0;; s++) {
+        for (uint32_t i = tid; i < P * 256; i += GRAM_BLOCK) {
+            const uint32_t p = i >> 8, d = i & 255u, lo = 8 * p / w;
+            if (8 * p >= h0 * w || 8 * p - w * lo != s) continue;
+            const uint32_t kb = w * (h0 - lo), wd = kb - s < 8 ? kb - s : 8u;
+            if (d >> wd) continue;
+            uint32_t sum = 0;
+            for (uint32_t y = d; y < len; y += 1u << wd) sum += h[y];
+            dh[p][d] = sum;
+        }
+        if (s == smax) break;
+        uint32_t pr[(1u << GRAM_MAXBITS) / 2 / GRAM_BLOCK];
+#pragma unroll
+        for (uint32_t k = 0; k < (1u << GRAM_MAXBITS) / 2 / GRAM_BLOCK; k++) {
+            const uint32_t y = tid + k * GRAM_BLOCK;
+            pr[k] = y < len / 2 ? h[2 * y] + h[2 * y + 1] : 0u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < (1u << GRAM_MAXBITS) / 2 / GRAM_BLOCK; k++) {
+            const uint32_t y = tid + k * GRAM_BLOCK;
+            if (y < len / 2) h[y] = pr[k];
+        }
+        __syncthreads();
+        len >>= 1;
+    }
+    __syncthreads();
+    if (r.tb == 0 && tid < P && 8 * tid < h0 * w) {   // end corrections of digit p (counts wrap; the sums are exact)
+        const uint32_t p = tid;
+        const int64_t o = (int64_t)h0 - (int64_t)(8 * p / w) - (int64_t)m, nn = n;
+        // positions read: below h0 <= 56 at the start, from n - 7 at the end
+        auto x = [&](int64_t q) -> uint32_t {
+            if (q < 0 || q >= nn) return 0;
+            return q < 64 ? s_end[0][q] : s_end[1][q - (nn - 64)];
+        };
+        auto G = [&](int64_t j) {
+            uint32_t g = 0;
+            for (uint32_t k = 0; k < m; k++) g = (g << w) | x(j + (int64_t)k);
+            return g;
+        };
+        if (o > 0) {   // suffixes i >= n - o read all-zero grams past the end; grams j < o belong to no suffix
+            const int64_t e = o < nn ? o : nn;
+            for (int64_t j = 0; j < e; j++) atomicSub(&dh[p][gram_digit(G(j), p, w, h0, m)], 1u);
+            atomicAdd(&dh[p][0], (uint32_t)e);
+        } else if (o < 0) {   // suffixes i < -o read grams starting before 0; grams j >= n + o belong to no suffix
+            for (int64_t j = nn + o > 0 ? nn + o : 0; j < nn; j++)
+                atomicSub(&dh[p][gram_digit(G(j), p, w, h0, m)], 1u);
+            const int64_t e = -o < nn ? -o : nn;
+            for (int64_t i = 0; i < e; i++) atomicAdd(&dh[p][gram_digit(G(i + o), p, w, h0, m)], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < P * 256; i += GRAM_BLOCK) {
+        const uint32_t v = (&dh[0][0])[i];
+        if (v) atomicAdd(&counts[(size_t)r.b * P * 256 + i], v);
+    }
 }
 
 // Digit histograms of stored keys at the given shifts (one flat range).
@@ -1135,6 +1351,10 @@ void launch_hist_T(const uint8_t *T, Tiling tl, uint32_t ntiles, const uint8_t *
                    hipStream_t s) {
     const uint32_t grid = ntiles < TEXT_HIST_GRID ? ntiles : TEXT_HIST_GRID;
     if (grid) k_hist_T<<<grid, 256, 0, s>>>(T, tl, ntiles, codes, P, counts);
+}
+void launch_hist_gram(const uint8_t *T, Tiling units, uint32_t nunits, const uint8_t *codes, const uint32_t *tot,
+                      uint32_t P, uint32_t *counts, hipStream_t s) {
+    if (nunits) k_hist_gram<<<nunits, GRAM_BLOCK, 0, s>>>(T, units, codes, tot, P, counts);
 }
 void launch_hist_keys(const uint64_t *K, uint32_t m, const Shifts &sh, uint32_t *counts, uint32_t grid,
                       hipStream_t s) {

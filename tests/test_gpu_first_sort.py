@@ -1,8 +1,9 @@
 """Shapes of the first sort's key space (GPU): runs of one byte inside random bytes (a run of R bytes gives about R rows
 whose first keys agree, one group the doubling rounds split), whole blocks of one or two byte values, keys of five
 8-bit digits (20- and 30-symbol alphabets, 5-bit codes), and such blocks mixed in one batch, where every block keeps its
-own code width; key widths of five, six and seven 8-bit digits. Bar: bit-exact against the oracle, and every block
-round-trips.
+own code width; key widths of five, six and seven 8-bit digits; blocks of every code width from 1 to 8 bits at lengths
+around the m-gram histograms' chunk (16 Ki positions) and unit (128 Ki), and tiny ones whose end corrections cover the
+whole block. Bar: bit-exact against the oracle, and every block round-trips.
 """
 import sys
 
@@ -44,6 +45,13 @@ def small_alphabet(n, sigma, seed):
     return (rng.choice(sigma, n, p=p) + 65).astype(np.uint8)
 
 
+def alphabet_of(n, sigma, seed):
+    """n bytes over sigma byte values spread over 0..255 (code width ceil(log2(sigma)) when all occur)."""
+    rng = np.random.default_rng(seed)
+    vals = np.sort(rng.permutation(256)[:sigma]).astype(np.uint8)
+    return vals[rng.integers(0, sigma, n)]
+
+
 def cases():
     yield "one byte", np.full(100000, 7, np.uint8)
     yield "two bytes", (np.random.default_rng(3).integers(0, 2, 300000) + 48).astype(np.uint8)
@@ -72,6 +80,39 @@ def test_gpu_first_sort_batch(gpu):
               np.frombuffer(b"ab", np.uint8).copy(), small_alphabet(70000, 30, 12)]
     rs = [1 << K.codec18_rlog(b.size) for b in blocks]
     say("first sort batch", [b.size for b in blocks])
+    Us, Is = K.bwt_aux_batch(blocks, rs)
+    for b, (T, r, U, I) in enumerate(zip(blocks, rs, Us, Is)):
+        eU, eI = refs.oracle_bwt(T, r)
+        assert np.array_equal(U, eU) and np.array_equal(I, eI), f"block {b}"
+        assert np.array_equal(K.unbwt_aux(U, r, I), T), f"block {b} inverse"
+
+
+GRAM_SIGMAS = (1, 2, 3, 4, 7, 16, 40, 64, 100, 128, 200, 256)   # code widths 1, 1, 2, 2, 3, 4, 6, 6, 7, 7, 8, 8
+
+
+def test_gpu_first_sort_gram_widths(gpu):
+    """The digit histograms from code m-grams, for every code width: chunk and unit edges, the end corrections."""
+    for sigma in GRAM_SIGMAS:
+        for n in (2, 3, 17, 65, 16383, 16385, 131071, 131073):
+            T = alphabet_of(n, sigma, 1000 * sigma + n)
+            say("gram widths", sigma, n)
+            check(T, f"sigma {sigma} n {n}")
+    for sigma in (4, 40, 100):
+        T = alphabet_of(2 * 131072 + 5, sigma, sigma)
+        say("gram widths", sigma, T.size)
+        check(T, f"sigma {sigma} n {T.size}")
+
+
+def test_gpu_first_sort_gram_batch(gpu):
+    """Blocks of every code width packed at odd offsets in one batch (m-gram units that start unaligned), with 5-bit
+    blocks (per-digit histograms) between them."""
+    rng = np.random.default_rng(77)
+    blocks = []
+    for k, sigma in enumerate(GRAM_SIGMAS + (20, 30)):
+        n = int(rng.integers(1000, 40000)) * 2 + 1 if k % 3 else 131072 + 2 * k + 1
+        blocks.append(alphabet_of(n, sigma, 500 + k))
+    rs = [1 << K.codec18_rlog(b.size) for b in blocks]
+    say("gram batch", [b.size for b in blocks])
     Us, Is = K.bwt_aux_batch(blocks, rs)
     for b, (T, r, U, I) in enumerate(zip(blocks, rs, Us, Is)):
         eU, eI = refs.oracle_bwt(T, r)
