@@ -936,7 +936,8 @@ __global__ __launch_bounds__(256) void k_bwt(const uint8_t *__restrict__ T, cons
                                              const uint32_t *__restrict__ ISA, const uint16_t *__restrict__ blk,
                                              const BlockInfo *__restrict__ blocks, uint8_t *__restrict__ U,
                                              uint32_t N) {
-    const size_t o0 = ((size_t)blockIdx.x * 256 + threadIdx.x) * 16;
+    constexpr int CH = 64;   // output bytes per thread: four 16-byte vectors
+    const size_t o0 = ((size_t)blockIdx.x * 256 + threadIdx.x) * CH;
     if (o0 >= N) return;
     auto info = [&](size_t o, uint32_t &off, uint32_t &n) {
         off = 0;
@@ -947,29 +948,53 @@ __global__ __launch_bounds__(256) void k_bwt(const uint8_t *__restrict__ T, cons
             n = bi.n;
         }
     };
-    const bool whole = o0 + 16 <= N && (reinterpret_cast<uintptr_t>(U + o0) & 15u) == 0 &&
-                       (!blk || blk[o0] == blk[o0 + 15]);
+    const bool whole = o0 + CH <= N && (reinterpret_cast<uintptr_t>(U + o0) & 15u) == 0 &&
+                       (!blk || blk[o0] == blk[o0 + CH - 1]);
     if (whole) {
+        // byte lo of the block is L[lo - 1] (shifted) for 1 <= lo <= p0 and L[lo] after the primary row: each output
+        // word is its own word of L, or that word shifted up one byte with the last byte of the word before it
         uint32_t off, n;
         info(o0, off, n);
         const uint32_t p0 = ISA[off] - off;
-        const uint4 cur = *reinterpret_cast<const uint4 *>(L + o0);   // L comes from hipMalloc: o0 is 16-aligned
-        const uint32_t w[4] = {cur.x, cur.y, cur.z, cur.w};
-        uint8_t lb[17];
-        lb[0] = o0 > 0 ? L[o0 - 1] : (uint8_t)0;
+        uint32_t w[CH / 4 + 1];   // w[0]: the word before the chunk (L comes from hipMalloc: o0 is 16-aligned)
+        w[0] = o0 >= 4 ? *reinterpret_cast<const uint32_t *>(L + o0 - 4) : 0u;
 #pragma unroll
-        for (int i = 0; i < 16; i++) lb[i + 1] = (uint8_t)(w[i >> 2] >> (8 * (i & 3)));
-        uint32_t ow[4] = {0, 0, 0, 0};
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const size_t lo = o0 + i - off;
-            const uint8_t c = lo == 0 ? T[off + n - 1] : (lo <= p0 ? lb[i] : lb[i + 1]);
-            ow[i >> 2] |= (uint32_t)c << (8 * (i & 3));
+        for (int q = 0; q < CH / 16; q++) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(L + o0 + 16 * q);
+            w[1 + 4 * q] = v.x;
+            w[2 + 4 * q] = v.y;
+            w[3 + 4 * q] = v.z;
+            w[4 + 4 * q] = v.w;
         }
-        *reinterpret_cast<uint4 *>(U + o0) = make_uint4(ow[0], ow[1], ow[2], ow[3]);
+        const uint8_t last = T[off + n - 1];
+        uint32_t ow[CH / 4];
+#pragma unroll
+        for (int k = 0; k < CH / 4; k++) {
+            const size_t lo = o0 + 4 * k - off;   // block position of the word's first byte
+            const uint32_t sh = __builtin_amdgcn_alignbyte(w[k + 1], w[k], 3);   // L[lo - 1 .. lo + 2]
+            uint32_t x;
+            if (lo + 3 <= p0 && lo > 0) {
+                x = sh;
+            } else if (lo > p0) {
+                x = w[k + 1];
+            } else {
+                x = 0;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const size_t b = lo + j;
+                    const uint32_t c = b == 0 ? last : (b <= p0 ? (sh >> (8 * j)) : (w[k + 1] >> (8 * j))) & 255u;
+                    x |= (c & 255u) << (8 * j);
+                }
+            }
+            ow[k] = x;
+        }
+#pragma unroll
+        for (int q = 0; q < CH / 16; q++)
+            *reinterpret_cast<uint4 *>(U + o0 + 16 * q) =
+                make_uint4(ow[4 * q], ow[4 * q + 1], ow[4 * q + 2], ow[4 * q + 3]);
         return;
     }
-    for (int i = 0; i < 16 && o0 + i < N; i++) {
+    for (int i = 0; i < CH && o0 + i < N; i++) {
         const size_t o = o0 + i;
         uint32_t off, n;
         info(o, off, n);
@@ -1426,7 +1451,7 @@ void launch_round_keys(const uint32_t *pos, const uint32_t *grp, uint32_t m, con
 }
 void launch_bwt(const uint8_t *T, const uint8_t *L, const uint32_t *ISA, const uint16_t *blk, const BlockInfo *blocks,
                 uint8_t *U, uint32_t N, hipStream_t s) {
-    if (N) k_bwt<<<(uint32_t)(((size_t)N + 4095) / 4096), 256, 0, s>>>(T, L, ISA, blk, blocks, U, N);
+    if (N) k_bwt<<<(uint32_t)(((size_t)N + 256 * 64 - 1) / (256 * 64)), 256, 0, s>>>(T, L, ISA, blk, blocks, U, N);
 }
 void launch_samples(const uint32_t *ISA, const BlockInfo *blocks, uint32_t B, uint32_t count, int32_t *I,
                     hipStream_t s) {
