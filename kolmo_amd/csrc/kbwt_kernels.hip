@@ -1267,7 +1267,8 @@ __global__ __launch_bounds__(256) void k_check_lists(const uint64_t *__restrict_
 // sublists at a time: each lane reads its sublist's 16-byte chunks of those steps (only chunks it staged), and each
 // wave then writes the bytes of one sublist at a time as one contiguous run. A sublist longer than cap walks on from
 // cont.
-constexpr int EMIT_ROWS = 64, EMIT_PITCH = WALK_TILE + 4;   // pitch: a wave's lanes read one column without conflicts
+constexpr int EMIT_ROWS = 64, EMIT_PITCH = EMIT_ROWS + 16;   // a sublist's 64 steps per LDS row; the pitch keeps
+                                                            // the lanes' 16-byte writes free of bank conflicts
 __global__ __launch_bounds__(WALK_TILE) void k_walk_emit(const uint32_t *__restrict__ lf, Tiling tl, uint32_t dshift,
                                                          uint32_t S, const uint64_t *__restrict__ link,
                                                          const uint32_t *__restrict__ len,
@@ -1276,8 +1277,7 @@ __global__ __launch_bounds__(WALK_TILE) void k_walk_emit(const uint32_t *__restr
                                                          uint32_t cap, const uint32_t *__restrict__ cont,
                                                          const uint32_t *__restrict__ bad, uint8_t *__restrict__ out) {
     __shared__ uint32_t s_c[256];
-    __shared__ uint32_t s_pos[WALK_TILE], s_st[WALK_TILE];
-    __shared__ uint8_t s_rows[EMIT_ROWS * EMIT_PITCH];
+    __shared__ __align__(16) uint8_t s_rows[WALK_TILE * EMIT_PITCH];   // [sublist][step of the window]
     __shared__ uint32_t s_max;
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
     const uint32_t b = tile_block(tl.first, tl.B, blockIdx.x);
@@ -1298,8 +1298,6 @@ __global__ __launch_bounds__(WALK_TILE) void k_walk_emit(const uint32_t *__restr
         if (pos > bi.n || steps > pos) steps = 0;           // only a string that is no BWT gets here
     }
     const uint32_t staged = steps < cap ? steps : cap;
-    s_pos[tid] = pos;
-    s_st[tid] = staged;
     __syncthreads();
     if (staged) atomicMax(&s_max, staged);
     __syncthreads();
@@ -1315,19 +1313,18 @@ __global__ __launch_bounds__(WALK_TILE) void k_walk_emit(const uint32_t *__restr
                 v[q] = t0 + 16 * q < staged ? *reinterpret_cast<const uint4 *>(src + (size_t)q * 16 * sstride)
                                             : make_uint4(0, 0, 0, 0);
 #pragma unroll
-            for (int q = 0; q < EMIT_ROWS / 16; q++) {
-                const uint32_t wd[4] = {v[q].x, v[q].y, v[q].z, v[q].w};
-#pragma unroll
-                for (int u = 0; u < 16; u++)
-                    if (16 * q + u < (int)nr) s_rows[(16 * q + u) * EMIT_PITCH + tid] = (uint8_t)(wd[u >> 2] >> (8 * (u & 3)));
-            }
+            for (int q = 0; q < EMIT_ROWS / 16; q++) *reinterpret_cast<uint4 *>(s_rows + tid * EMIT_PITCH + 16 * q) = v[q];
         }
         __syncthreads();
-        for (int k = wave * WAVE; k < wave * WAVE + WAVE; k++) {
-            const uint32_t sk = s_st[k];
-            if (sk <= t0) continue;
-            const uint32_t t = t0 + (uint32_t)lane;
-            if (t < sk && (uint32_t)lane < nr) out[bi.off + s_pos[k] - 1 - t] = s_rows[lane * EMIT_PITCH + k];
+        // sublist k of the wave is lane k's: its staged steps and position come from that lane's registers as scalars
+        uint8_t *const ob = out + bi.off - 1;
+        const uint32_t t = t0 + (uint32_t)lane;
+        const bool row_ok = (uint32_t)lane < nr;
+        // only the wave's sublists with steps in this window
+        for (unsigned long long act = __ballot(staged > t0); act; act &= act - 1) {
+            const int k = __builtin_ctzll(act);
+            const uint32_t sk = __builtin_amdgcn_readlane(staged, k), pk = __builtin_amdgcn_readlane(pos, k);
+            if (t < sk && row_ok) ob[pk - t] = s_rows[(wave * WAVE + k) * EMIT_PITCH + lane];
         }
         __syncthreads();
     }
