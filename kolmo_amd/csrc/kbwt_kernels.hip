@@ -695,7 +695,18 @@ __global__ __launch_bounds__(256) void k_regroup_carry(const uint64_t *__restric
     if (t >= ntiles) return;
     const uint32_t a0 = t * (uint32_t)RG_TILE;
     const uint64_t v = K[a0] >> kshift;
-    uint32_t lo = blk ? blocks[blk[a0]].off : 0u, hi = a0;   // first row in [lo, a0] whose key is >= v
+    const uint32_t base = blk ? blocks[blk[a0]].off : 0u;
+    // the first row in [base, a0] whose key is >= v lies in [lo, hi], and K[hi] >= v. Galloping down from a0 first:
+    // the group mostly starts at a0 or a few rows before it, so few of the binary search's dependent loads remain
+    uint32_t lo = base, hi = a0;
+    for (uint32_t step = 1; hi > base; step <<= 1) {
+        const uint32_t probe = hi - base > step ? hi - step : base;
+        if ((K[probe] >> kshift) < v) {
+            lo = probe + 1;
+            break;
+        }
+        hi = probe;
+    }
     while (lo < hi) {
         const uint32_t mid = lo + ((hi - lo) >> 1);
         if ((K[mid] >> kshift) < v) lo = mid + 1; else hi = mid;
