@@ -135,18 +135,37 @@ __device__ __forceinline__ uint32_t count_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Bit B of d spread over a word (all ones when set): one signed bit-field extract. The compiler turns the equivalent
+// shift pair back into two shifts and a sign test, so the instruction is written out.
+template <int B> __device__ __forceinline__ uint32_t bit_mask(uint32_t d) {
+    uint32_t s;
+    asm("v_bfe_i32 %0, %1, %2, 1" : "=v"(s) : "v"(d), "i"(B));
+    return s;
+}
+// acc | (m ^ s) in one three-input logic operation: truth table 0xF6 over (acc, m, s), indexed acc*4 + m*2 + s.
+__device__ __forceinline__ uint32_t or_xor(uint32_t acc, uint32_t m, uint32_t s) {
+    return __builtin_amdgcn_bitop3_b32(acc, m, s, (unsigned char)0xF6);
+}
+template <int B> __device__ __forceinline__ void match_bit(uint32_t d, uint32_t &dlo, uint32_t &dhi) {
+    const uint32_t s = bit_mask<B>(d);
+    const uint64_t m = __ballot(s != 0u);
+    dlo = or_xor(dlo, (uint32_t)m, s);
+    dhi = or_xor(dhi, (uint32_t)(m >> 32), s);
+}
 // Peers of this lane: the valid lanes whose 8-bit digit equals d. valid_mask = ballot(valid). One ballot per digit
 // bit; the lanes that differ from this one in bit b are that ballot XOR the lane's own bit spread over a word, and
-// the peers are the valid lanes that differ in no bit (six vector instructions per bit).
+// the peers are the valid lanes that differ in no bit (an extract, a compare and two three-input logic operations per
+// bit).
 __device__ __forceinline__ uint64_t match_digit8(uint32_t d, uint64_t valid_mask) {
     uint32_t dlo = 0, dhi = 0;
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-        const uint32_t s = (uint32_t)((int32_t)(d << (31 - b)) >> 31);   // all ones when bit b of d is set
-        const uint64_t m = __ballot(s != 0u);
-        dlo |= (uint32_t)m ^ s;
-        dhi |= (uint32_t)(m >> 32) ^ s;
-    }
+    match_bit<0>(d, dlo, dhi);
+    match_bit<1>(d, dlo, dhi);
+    match_bit<2>(d, dlo, dhi);
+    match_bit<3>(d, dlo, dhi);
+    match_bit<4>(d, dlo, dhi);
+    match_bit<5>(d, dlo, dhi);
+    match_bit<6>(d, dlo, dhi);
+    match_bit<7>(d, dlo, dhi);
     return valid_mask & ~(((uint64_t)dhi << 32) | dlo);
 }
 
