@@ -128,27 +128,45 @@ __global__ __launch_bounds__(256) void k_byte_hist(const uint8_t *__restrict__ T
         if (c) atomicAdd(&counts[(size_t)cur * 256 + tid], c);
         __syncthreads();
     };
-    for (uint32_t t = t0; t < t1; t++) {
-        TileRange r = tile_range(tl, t, HIST_TILE, 0);
-        if ((int)r.b != cur) {
-            if (cur >= 0) flush();
-            cur = (int)r.b;
-        }
-        // 16-byte loads over the tile's aligned middle, single bytes for the unaligned head and tail
+    // A tile is at most one 16-byte vector per thread plus single head and tail bytes: each thread loads its share of
+    // the next tile before it counts the current one, so the loads' latency overlaps the atomics.
+    static_assert(HIST_TILE == 256 * 16, "one vector per thread and tile");
+    struct Share {
+        uint4 v;
+        uint32_t nv, head, tail, b;   // bytes of v that count; head/tail byte + 256 when present; block
+    };
+    auto fetch = [&](uint32_t t) {
+        Share sh{make_uint4(0, 0, 0, 0), 0u, 0u, 0u, 0u};
+        const TileRange r = tile_range(tl, t, HIST_TILE, 0);
+        sh.b = r.b;
         const size_t mis = (size_t)(reinterpret_cast<uintptr_t>(T + r.beg) & 15u);
         const size_t ab = r.beg + (mis ? 16 - mis : 0) < r.end ? r.beg + (mis ? 16 - mis : 0) : r.end;
         const size_t ae = ab + ((r.end - ab) & ~(size_t)15);
-        const uint4 *vp = reinterpret_cast<const uint4 *>(T + ab);   // 16-byte aligned address
-        for (size_t q16 = tid; q16 < ((ae - ab) >> 4); q16 += 256) {
-            const uint4 v = vp[q16];
-            const uint32_t wd[4] = {v.x, v.y, v.z, v.w};
+        if ((size_t)tid < ((ae - ab) >> 4)) {
+            sh.v = reinterpret_cast<const uint4 *>(T + ab)[tid];   // 16-byte aligned address
+            sh.nv = 16;
+        }
+        if ((size_t)tid < ab - r.beg) sh.head = 256u + T[r.beg + tid];
+        if ((size_t)tid < r.end - ae) sh.tail = 256u + T[ae + tid];
+        return sh;
+    };
+    Share sh = t0 < t1 ? fetch(t0) : Share{};
+    for (uint32_t t = t0; t < t1; t++) {
+        const Share cur_sh = sh;
+        if (t + 1 < t1) sh = fetch(t + 1);
+        if ((int)cur_sh.b != cur) {
+            if (cur >= 0) flush();
+            cur = (int)cur_sh.b;
+        }
+        if (cur_sh.nv) {
+            const uint32_t wd[4] = {cur_sh.v.x, cur_sh.v.y, cur_sh.v.z, cur_sh.v.w};
 #pragma unroll
             for (int q = 0; q < 4; q++)
 #pragma unroll
                 for (int k = 0; k < 4; k++) atomicAdd(&h[cp][(wd[q] >> (8 * k)) & 255u], 1u);
         }
-        if ((size_t)tid < ab - r.beg) atomicAdd(&h[cp][T[r.beg + tid]], 1u);
-        if ((size_t)tid < r.end - ae) atomicAdd(&h[cp][T[ae + tid]], 1u);
+        if (cur_sh.head) atomicAdd(&h[cp][cur_sh.head - 256u], 1u);
+        if (cur_sh.tail) atomicAdd(&h[cp][cur_sh.tail - 256u], 1u);
     }
     if (cur >= 0) flush();
 }
