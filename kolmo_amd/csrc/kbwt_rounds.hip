@@ -318,34 +318,53 @@ __global__ __launch_bounds__(256) void k_seg_round(const uint32_t *__restrict__ 
         return;
     }
 
-    // The sort key packs (group, sec, row): the row's slot in the low 12 bits (carried, not sorted), above it the sec
-    // bits that vary over the tile (the bits above the highest varying one are equal in every row, so dropping them
-    // keeps the order), and the group right above those. Only the digits of the sec and group bits are sorted.
-    uint32_t so = 0, sa = ~0u;
+    // The sort key packs (group, value, row): the row's slot in the low 12 bits (carried, not sorted), above it a
+    // value that orders and ties the rows of a group as their secs do, and the group right above it. The value is the
+    // sec itself with only the bits that vary over the tile (the bits above the highest varying one are equal in every
+    // row, so dropping them keeps the order), or the sec minus its group's smallest one when that needs fewer bits:
+    // on repetitive text the secs of one group lie close together while the tile's span the whole rank range. Only the
+    // digits of the value and group bits are sorted.
+    uint32_t *const s_gmin = &s_wcnt[0][0];   // the radix counters are free until the first pass clears them
+    static_assert(W * 256 >= MAXG, "group minima in the counters' LDS");
+    for (uint32_t j = tid; j < ng; j += 256) s_gmin[j] = ~0u;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++)
+        if (tid + k * 256 < cnt) atomicMin(&s_gmin[ga[k]], gv[k]);
+    __syncthreads();
+    uint32_t so = 0, sa = ~0u, rmax = 0;
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const uint32_t e = tid + k * 256;
         if (e < cnt) {
             so |= gv[k];
             sa &= gv[k];
+            rmax = max(rmax, gv[k] - s_gmin[ga[k]]);
         }
     }
     for (int o = 1; o < WAVE; o <<= 1) {
         so |= __shfl_xor(so, o, WAVE);
         sa &= __shfl_xor(sa, o, WAVE);
+        rmax = max(rmax, (uint32_t)__shfl_xor(rmax, o, WAVE));
     }
-    if (lane == 0) { s_or[wave] = so; s_and[wave] = sa; }
+    __shared__ uint32_t s_rmax[W];
+    if (lane == 0) { s_or[wave] = so; s_and[wave] = sa; s_rmax[wave] = rmax; }
     __syncthreads();
     const uint32_t svary =
         (uint32_t)((s_or[0] | s_or[1] | s_or[2] | s_or[3]) ^ (s_and[0] & s_and[1] & s_and[2] & s_and[3]));
-    const uint32_t sbits = svary ? 32u - (uint32_t)__clz(svary) : 0u;               // sec bits kept
+    const uint32_t rtop = max(max(s_rmax[0], s_rmax[1]), max(s_rmax[2], s_rmax[3]));
+    const uint32_t tbits = svary ? 32u - (uint32_t)__clz(svary) : 0u;               // varying sec bits of the tile
+    const uint32_t rbits = rtop ? 32u - (uint32_t)__clz(rtop) : 0u;                 // bits of the widest group span
+    const bool rel = rbits < tbits;
+    const uint32_t sbits = rel ? rbits : tbits;                                      // value bits
     const uint32_t gbits = ng > 1 ? 32u - (uint32_t)__clz(ng - 1u) : 0u;            // group bits
     const uint64_t smask = (1ull << sbits) - 1ull;
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         const uint32_t e = tid + k * 256;
         if (e < cnt) {
-            s_buf[e] = (((uint64_t)ga[k] << sbits) | ((uint64_t)gv[k] & smask)) << 12 | e;
+            const uint32_t val = rel ? gv[k] - s_gmin[ga[k]] : gv[k];
+            s_buf[e] = (((uint64_t)ga[k] << sbits) | ((uint64_t)val & smask)) << 12 | e;
             s_i[e] = gi[k];
             if (CARRY) s_lb[e] = gb[k];
         }
