@@ -915,8 +915,9 @@ __global__ __launch_bounds__(256) void k_isa_scatter(const uint64_t *__restrict_
 
 // ------------------------------------------------------------------ doubling round keys
 
-// key = (group head << 32) | sec, sec = ISA[i+h] + h when i+h lies inside the block, else end-1-i (below every rank,
-// shorter suffix first). orand[0] |= keys, orand[1] &= keys (which digits vary): one pair of atomics per workgroup.
+// key = (group << 32) | sec (group: its index in the head-sorted list), sec = ISA[i+h] + h when i+h lies inside
+// the block, else end-1-i (below every rank, shorter suffix first). orand[0] |= keys, orand[1] &= keys (which
+// digits vary): one pair of atomics per workgroup.
 __global__ __launch_bounds__(256) void k_round_keys(const uint32_t *__restrict__ pos, const uint32_t *__restrict__ grp,
                                                     uint32_t m, const uint32_t *__restrict__ SA,
                                                     const uint32_t *__restrict__ ISA, uint32_t n,

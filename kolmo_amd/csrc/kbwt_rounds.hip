@@ -84,7 +84,8 @@ __global__ __launch_bounds__(256) void k_scan_groups(const uint32_t *__restrict_
     }
 }
 
-// Rows of the large groups as element lists for the global path: pos = head + k, grp = head, in list order.
+// Rows of the large groups as element lists for the global path: pos = head + k, grp = the group's index in the
+// list (sorted by head), in list order.
 // lo[g] = first row of large group g in the element list (host prefix sums), G large groups, m rows.
 __global__ __launch_bounds__(256) void k_large_elems(const uint32_t *__restrict__ lh, const uint32_t *__restrict__ lo,
                                                      uint32_t G, uint32_t m, uint32_t *__restrict__ pos,
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(256) void k_large_elems(const uint32_t *__restrict_
     for (size_t a = (size_t)blockIdx.x * 256 + threadIdx.x; a < m; a += (size_t)gridDim.x * 256) {
         uint32_t g = lower_bound_u32(lo, G, (uint32_t)a + 1) - 1;   // last g with lo[g] <= a
         pos[a] = lh[g] + ((uint32_t)a - lo[g]);
-        grp[a] = lh[g];
+        grp[a] = g;   // the list is in head order, so the index orders groups as their heads do, in fewer bits
     }
 }
 
