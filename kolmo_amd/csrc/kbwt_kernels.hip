@@ -6,8 +6,9 @@
 // Forward, per batch:
 //   1. k_byte_hist + k_scan_digits: byte histogram of every block; the host turns each into a dense order-preserving
 //      symbol code of w = ceil(log2 sigma) bits and picks h0 symbols per first key (h0*w <= 56 bits).
-//   2. k_hist_T + k_scan_digits: the 8-bit digit histograms of every first key of every block, in one read of T;
-//      exclusive scans give each block's digit buckets (onesweep).
+//   2. k_hist_gram (and k_hist_T for 5-bit codes) + k_scan_digits: the 8-bit digit histograms of every first key of
+//      every block, in one read of T (none for 8-bit codes: their byte histogram serves), from one count per position
+//      of the codes' m-grams; exclusive scans give each block's digit buckets (onesweep).
 //   3. k_sort_pass: one stable LSD pass per digit, segmented by block. The first pass builds the keys from T: the
 //      symbol codes above the BWT byte T[i-1]. A tile of BLOCK*ITEMS elements is ranked by wave64 ballot multisplit,
 //      its digit offsets come from a decoupled lookback over earlier tiles of its block, and it leaves through LDS
@@ -18,9 +19,9 @@
 //      window in the first round), the row's BWT byte, and the groups of two or more appended to the next round's
 //      group lists.
 //   5. per doubling round with offset h: groups of at most SEG_GROUP rows sort inside one workgroup each
-//      (kbwt_rounds.hip); for the rows of larger groups k_round_keys gathers (group head, ISA[i+h] + h, or the
-//      distance to the block's end when i+h lies past it), the same radix passes sort them (one flat range: the heads
-//      already keep blocks apart), and k_regroup splits those groups.
+//      (kbwt_rounds.hip); for the rows of larger groups k_round_keys gathers (the group's index in the head-sorted
+//      list, ISA[i+h] + h, or the distance to the block's end when i+h lies past it), the same radix passes sort them
+//      (one flat range: the group indexes already keep blocks apart), and k_regroup splits those groups.
 //   6. k_bwt + k_samples: U in libsais' convention per block and I[k] = 1 + rank of suffix k*r.
 // Inverse, per batch of BWTs L with primary indexes P (the end marker's row in the n+1-row matrix of each block):
 //   1. byte histograms -> C[c] = first row of byte c, as a global row.
