@@ -764,6 +764,8 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
     }
     // digit histograms of the first keys: m-gram units (8-bit codes: one unit, from the byte counts) or, for 5-bit
     // codes, k_hist_T's tiles
+    uint32_t maxn = 0;   // the largest block (the ISA scatter's runs follow its windows)
+    for (uint32_t b = 0; b < B; b++) maxn = bl[b].n > maxn ? bl[b].n : maxn;
     bool plain_hist = false;
     const uint32_t ngu = fill_tiles((uint32_t *)(hm + o_gu), B, [&](uint32_t b) {
         if (!gram_block(bl[b].w)) {
@@ -860,14 +862,14 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
               launch_partition_pairs(K[P0 & 1], K[(P0 - 1) & 1], N, tpart, npt, P<uint64_t>(w.status),
                                      w.counters + 4, MAXP, depoch, s));
         LAUNCHCHK();
-        TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, ISA, s));
+        TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, maxn, ISA, s));
         LAUNCHCHK();
     };
     if (feat) {   // the router's path: its per-round counts are set up around this sequence, so no graph
         first_sort();
     } else {
         const std::vector<uint64_t> key = {
-            B, N, P0, (uint64_t)scfg, nht, nst, npt, ngu, (uint64_t)plain_hist, (uintptr_t)w.tot.p, (uintptr_t)dT,
+            B, N, P0, (uint64_t)scfg, nht, nst, npt, ngu, (uint64_t)plain_hist, maxn, (uintptr_t)w.tot.p, (uintptr_t)dT,
             (uintptr_t)dm, (uintptr_t)w.counts.p,
             (uintptr_t)w.dbase.p, (uintptr_t)w.K0.p, (uintptr_t)w.K1.p, (uintptr_t)w.V0.p, (uintptr_t)w.V1.p,
             (uintptr_t)w.SA.p, (uintptr_t)w.ISA.p, (uintptr_t)w.L.p, (uintptr_t)w.carry.p, (uintptr_t)w.gh[0].p,

@@ -894,14 +894,15 @@ __global__ __launch_bounds__(BLOCK) void k_regroup(const uint64_t *__restrict__ 
 
 // pairs: (suffix << 32 | head) partitioned by suffix window inside each block, so that every window occupies the
 // positions of its own suffix range (the suffixes of a block are a permutation of its index range): the pairs at
-// positions [p, q) carry suffixes of the windows that overlap [p, q). ISA[suffix] = head. A run of SC_SUPER tiles
+// positions [p, q) carry suffixes of the windows that overlap [p, q). ISA[suffix] = head. A run of `super` tiles
 // therefore writes a short stretch of ISA, and each run is written by one XCD (workgroups are dealt to the XCDs
-// round-robin by id), whose L2 merges the writes into whole lines.
-constexpr int SC_ITEMS = 16, SC_TILE = 256 * SC_ITEMS, SC_SUPER = 32, NXCD = 8;
+// round-robin by id), whose L2 merges the writes into whole lines. A run covers a whole window of the largest block
+// (at most 2 MB of ISA), so that no window's lines are written in part by several XCDs' L2s.
+constexpr int SC_ITEMS = 16, SC_TILE = 256 * SC_ITEMS, NXCD = 8;
 __global__ __launch_bounds__(256) void k_isa_scatter(const uint64_t *__restrict__ pairs, uint32_t N,
-                                                     uint32_t *__restrict__ ISA) {
+                                                     uint32_t *__restrict__ ISA, uint32_t super) {
     const uint32_t x = blockIdx.x % NXCD, k = blockIdx.x / NXCD;
-    const size_t chunk = ((size_t)(k / SC_SUPER) * NXCD + x) * SC_SUPER + k % SC_SUPER;
+    const size_t chunk = ((size_t)(k / super) * NXCD + x) * super + k % super;
     const size_t b0 = chunk * SC_TILE + threadIdx.x;
     uint64_t v[SC_ITEMS];
 #pragma unroll
@@ -1455,12 +1456,16 @@ void launch_partition_pairs(const uint64_t *in, uint64_t *out, uint32_t N, const
     k_sort_pass<PART_BLOCK, PART_ITEMS, 4, false, false><<<ntiles, PART_BLOCK, 0, s>>>(
         in, nullptr, out, nullptr, N, 0, nullptr, 0, tl, status, counter, epoch, depoch, ntiles, nullptr, nullptr);
 }
-void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t *ISA, hipStream_t s) {
+void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t maxn, uint32_t *ISA, hipStream_t s) {
     if (!N) return;
+    // the largest block's window (256 windows per block, as the ISA partition cuts them), in tiles: 32 to 128
+    const uint32_t ws = maxn > 256 ? 24u - (uint32_t)__builtin_clz(maxn - 1u) : 0u;
+    const uint32_t wt = (1u << ws) / SC_TILE;
+    const uint32_t super = wt < 32u ? 32u : (wt > 128u ? 128u : wt);
     const size_t chunks = ((size_t)N + SC_TILE - 1) / SC_TILE;
-    const size_t runs = (chunks + SC_SUPER - 1) / SC_SUPER;
-    const size_t grid = (runs + NXCD - 1) / NXCD * NXCD * SC_SUPER;
-    k_isa_scatter<<<(uint32_t)grid, 256, 0, s>>>(pairs, N, ISA);
+    const size_t runs = (chunks + super - 1) / super;
+    const size_t grid = (runs + NXCD - 1) / NXCD * NXCD * super;
+    k_isa_scatter<<<(uint32_t)grid, 256, 0, s>>>(pairs, N, ISA, super);
 }
 void launch_regroup(const RegroupArgs &a, hipStream_t s) {
     const uint32_t ntiles = (a.m + RG_TILE - 1) / RG_TILE;
