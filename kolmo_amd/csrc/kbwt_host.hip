@@ -567,7 +567,8 @@ constexpr int TUNE_CLASSES = 33;
 // The built-in table, measured on one MI355X over single text + CSV blocks of 1-32 MiB: up to 2 MiB the radix pass
 // takes 8 K-key tiles (more workgroups than CUs) and the inverse walks shorter sublists (splitters every 8 or 16
 // rows: the longest sublist bounds the walk); to 8 MiB splitters every 16 rows; from 16 MiB on 16 K-key tiles and
-// splitters every 64 rows.
+// splitters every 64 rows; above 64 MiB every 128 rows (the sublists' links then stay within the caches: on a 128 MiB
+// block the ranking takes 0.32 ms instead of 0.83 for a walk 0.16 ms longer).
 Tuning default_tuning(uint32_t cls) {
     Tuning t;
     if (cls <= 20) {
@@ -578,6 +579,8 @@ Tuning default_tuning(uint32_t cls) {
         t.inv_dshift = 4;
     } else if (cls <= 23) {
         t.inv_dshift = 4;
+    } else if (cls >= 27) {
+        t.inv_dshift = 7;
     }
     return t;
 }
