@@ -295,10 +295,13 @@ Workspace &workspace() {
             const uint32_t e0 = EPOCH_HALF;
             HIPCHK(hipMemcpy(w.counters + 15, &e0, 4, hipMemcpyHostToDevice));
         }
-        // timing only, read after the stream is synchronised: no system-scope fence (a fence writes back and
-        // invalidates the caches at every record, which slows the work around it)
-        HIPCHK(hipEventCreateWithFlags(&w.ev0, hipEventDisableSystemFence));
-        HIPCHK(hipEventCreateWithFlags(&w.ev1, hipEventDisableSystemFence));
+        // Events with the default system-scope fence. A call records ev0 right after copying its input and metadata
+        // to the device, and that record's fence is what invalidates the XCDs' L2 caches before the first kernel
+        // reads the copied bytes. Without it, lines of the same buffer cached by an earlier call (the previous
+        // call's decoded output, when the input buffer comes back at the same address) were read in place of the
+        // new input now and then, and a block of the batch came out wrong.
+        HIPCHK(hipEventCreate(&w.ev0));
+        HIPCHK(hipEventCreate(&w.ev1));
         w.ready = true;
     }
     return w;
@@ -516,9 +519,9 @@ template <class F> void timed(Workspace &w, hipStream_t s, const char *name, uin
         w.spare.pop_back();
         b = w.spare.back();
         w.spare.pop_back();
-    } else {   // timing only (prof_collect reads them after a synchronisation): no system-scope fence
-        HIPCHK(hipEventCreateWithFlags(&a, hipEventDisableSystemFence));
-        HIPCHK(hipEventCreateWithFlags(&b, hipEventDisableSystemFence));
+    } else {   // default fences: a pair may be recorded right after a host-to-device copy (see ev0)
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
     }
     HIPCHK(hipEventRecord(a, s));
     fn();
@@ -865,7 +868,7 @@ void forward(Workspace &w, const uint8_t *dT, uint8_t *dU, std::vector<BlockInfo
               launch_partition_pairs(K[P0 & 1], K[(P0 - 1) & 1], N, tpart, npt, P<uint64_t>(w.status),
                                      w.counters + 4, MAXP, depoch, s));
         LAUNCHCHK();
-        TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, maxn, ISA, s));
+        TIMED("isa_scatter", N, launch_isa_scatter(K[(P0 - 1) & 1], N, maxn, blk, d_bl, ISA, s));
         LAUNCHCHK();
     };
     if (feat) {   // the router's path: its per-round counts are set up around this sequence, so no graph

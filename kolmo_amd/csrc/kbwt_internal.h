@@ -144,7 +144,8 @@ constexpr int PART_BLOCK = 1024, PART_ITEMS = 16, PART_TILE = PART_BLOCK * PART_
 void launch_partition_pairs(const uint64_t *in, uint64_t *out, uint32_t N, const Tiling &tl, uint32_t ntiles,
                             uint64_t *status, uint32_t *counter, uint32_t epoch, const uint32_t *depoch,
                             hipStream_t s);
-void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t maxn, uint32_t *ISA, hipStream_t s);
+void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t maxn, const uint16_t *blk,
+                        const BlockInfo *blocks, uint32_t *ISA, hipStream_t s);
 void launch_round_keys(const uint32_t *pos, const uint32_t *grp, uint32_t m, const uint32_t *SA, const uint32_t *ISA,
                        uint32_t n, const uint16_t *blk, const BlockInfo *blocks, uint32_t round, uint64_t *K, uint32_t *V,
                        unsigned long long *orand, hipStream_t s);

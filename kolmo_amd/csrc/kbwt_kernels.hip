@@ -15,9 +15,9 @@
 //      so that equal digits go out as contiguous runs.
 //   4. k_regroup_carry + k_regroup: group heads (rank = SA position of the group's first row) by a max-scan inside
 //      each tile, whose carry-in (the head of the group holding the tile's first row) comes from a binary search over
-//      the sorted keys, so tiles need nothing from each other; ISA[suffix] = head (through pairs partitioned by ISA
-//      window in the first round), the row's BWT byte, and the groups of two or more appended to the next round's
-//      group lists.
+//      the sorted keys, so tiles need nothing from each other; ISA[suffix] = head (in the first round through pairs
+//      partitioned by ISA window, whose stretches of ISA are assembled in LDS or, for wide windows, scattered run by
+//      run), the row's BWT byte, and the groups of two or more appended to the next round's group lists.
 //   5. per doubling round with offset h: groups of at most SEG_GROUP rows sort inside one workgroup each
 //      (kbwt_rounds.hip); for the rows of larger groups k_round_keys gathers (the group's index in the head-sorted
 //      list, ISA[i+h] + h, or the distance to the block's end when i+h lies past it), the same radix passes sort them
@@ -915,6 +915,64 @@ __global__ __launch_bounds__(256) void k_isa_scatter(const uint64_t *__restrict_
         if (v[u] != ~0ull) ISA[(uint32_t)(v[u] >> 32)] = (uint32_t)v[u];
 }
 
+// The same writes assembled in LDS: workgroup j owns the suffixes [lo, lo + ISA_SUB), lo = j * ISA_SUB, of the
+// packed range. Its pairs all sit in the windows that overlap those suffixes (window d of block b holds the
+// positions of its own suffix range), so it scans the positions of those windows, keeps the pairs whose suffix is
+// its own, places each head in LDS and writes its stretch of ISA with 16-byte stores: every HBM line is written
+// once and whole, with no partial-line writes for the L2 to merge. A window wider than ISA_SUB is scanned by each
+// of the `super` workgroups that share it; they are dealt to one XCD one after the other (as in k_isa_scatter),
+// so that its L2 can serve part of the re-reads.
+constexpr int ISA_SUB = 32768, ISA_BLOCK = 1024, ISA_UNROLL = 4;
+constexpr uint32_t ISA_LDS_SHARE = 4;   // most workgroups that scan one window
+__global__ __launch_bounds__(ISA_BLOCK) void k_isa_lds(const uint64_t *__restrict__ pairs, uint32_t N,
+                                                       const uint16_t *__restrict__ blk,
+                                                       const BlockInfo *__restrict__ blocks,
+                                                       uint32_t *__restrict__ ISA, uint32_t super) {
+    __shared__ uint32_t s_isa[ISA_SUB];
+    const uint32_t x = blockIdx.x % NXCD, k = blockIdx.x / NXCD;
+    const size_t j = ((size_t)(k / super) * NXCD + x) * super + k % super;
+    if (j * ISA_SUB >= N) return;
+    const uint32_t lo = (uint32_t)j * ISA_SUB, cnt = N - lo < (uint32_t)ISA_SUB ? N - lo : (uint32_t)ISA_SUB;
+    // positions [p0, p1): from the start of the window of suffix lo to the end of the window of suffix lo + cnt - 1
+    const BlockInfo b0 = blocks[blk ? blk[lo] : 0u], b1 = blocks[blk ? blk[lo + cnt - 1] : 0u];
+    const uint32_t s0 = window_shift(b0.n), s1 = window_shift(b1.n);
+    const uint32_t p0 = (b0.off + ((lo - b0.off) >> s0 << s0)) & ~1u;   // even: two pairs per 16-byte load
+    const uint64_t e1 = (uint64_t)b1.off + ((uint64_t)((lo + cnt - 1 - b1.off) >> s1) + 1) * (1ull << s1);
+    const uint32_t p1 = (uint32_t)(e1 < (uint64_t)b1.off + b1.n ? e1 : (uint64_t)b1.off + b1.n);
+    const int tid = threadIdx.x;
+    // a pair outside [p0, p1) or of another window has its suffix outside [lo, lo + cnt) and is skipped
+    for (uint32_t q0 = p0 + 2u * tid; q0 < p1; q0 += 2u * ISA_BLOCK * ISA_UNROLL) {
+        uint64_t a[ISA_UNROLL], b[ISA_UNROLL];
+#pragma unroll
+        for (int u = 0; u < ISA_UNROLL; u++) {
+            const uint32_t q = q0 + 2u * ISA_BLOCK * u;
+            a[u] = b[u] = ~0ull;
+            if (q + 1 < p1) {
+                const ulonglong2 v = *reinterpret_cast<const ulonglong2 *>(pairs + q);
+                a[u] = v.x;
+                b[u] = v.y;
+            } else if (q < p1) {
+                a[u] = pairs[q];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < ISA_UNROLL; u++) {
+            uint32_t r = (uint32_t)(a[u] >> 32) - lo;
+            if (r < cnt) s_isa[r] = (uint32_t)a[u];
+            r = (uint32_t)(b[u] >> 32) - lo;
+            if (r < cnt) s_isa[r] = (uint32_t)b[u];
+        }
+    }
+    __syncthreads();
+    uint32_t *dst = ISA + lo;
+    if (cnt == (uint32_t)ISA_SUB) {
+        for (int i = tid; i < ISA_SUB / 4; i += ISA_BLOCK)
+            reinterpret_cast<uint4 *>(dst)[i] = reinterpret_cast<const uint4 *>(s_isa)[i];
+    } else {
+        for (uint32_t i = tid; i < cnt; i += ISA_BLOCK) dst[i] = s_isa[i];
+    }
+}
+
 // ------------------------------------------------------------------ doubling round keys
 
 // key = (group << 32) | sec (group: its index in the head-sorted list), sec = ISA[i+h] + h when i+h lies inside
@@ -1456,10 +1514,26 @@ void launch_partition_pairs(const uint64_t *in, uint64_t *out, uint32_t N, const
     k_sort_pass<PART_BLOCK, PART_ITEMS, 4, false, false><<<ntiles, PART_BLOCK, 0, s>>>(
         in, nullptr, out, nullptr, N, 0, nullptr, 0, tl, status, counter, epoch, depoch, ntiles, nullptr, nullptr);
 }
-void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t maxn, uint32_t *ISA, hipStream_t s) {
+void launch_isa_scatter(const uint64_t *pairs, uint32_t N, uint32_t maxn, const uint16_t *blk,
+                        const BlockInfo *blocks, uint32_t *ISA, hipStream_t s) {
     if (!N) return;
-    // the largest block's window (256 windows per block, as the ISA partition cuts them), in tiles: 32 to 128
+    // the largest block's window (256 windows per block, as the ISA partition cuts them)
     const uint32_t ws = maxn > 256 ? 24u - (uint32_t)__builtin_clz(maxn - 1u) : 0u;
+    // Windows of up to four workgroups' suffixes are assembled in LDS. Each of a window's workgroups reads all of
+    // its pairs, and the L2 serves few of the re-reads (the 32 workgroups of an XCD stream eight or more windows at
+    // once). Up to four per window it still gains (0.198 -> 0.186 ms on a 32 MiB block, four per window; 7.26 ->
+    // 3.63 ms per step on 512 blocks of up to 16 MiB, two at most); at sixteen the re-reads cost more than the
+    // partial-line writes (1.31 -> 2.40 ms on a 128 MiB block). Wider windows keep the scatter below.
+    if ((1u << ws) <= ISA_LDS_SHARE * (uint32_t)ISA_SUB) {
+        // workgroups per window (1 when a window is no wider than a workgroup's suffixes)
+        const uint32_t super = (1u << ws) > (uint32_t)ISA_SUB ? (1u << ws) / ISA_SUB : 1u;
+        const size_t subs = ((size_t)N + ISA_SUB - 1) / ISA_SUB;
+        const size_t runs = (subs + super - 1) / super;
+        const size_t grid = (runs + NXCD - 1) / NXCD * NXCD * super;
+        k_isa_lds<<<(uint32_t)grid, ISA_BLOCK, 0, s>>>(pairs, N, blk, blocks, ISA, super);
+        return;
+    }
+    // runs of 32 to 128 tiles
     const uint32_t wt = (1u << ws) / SC_TILE;
     const uint32_t super = wt < 32u ? 32u : (wt > 128u ? 128u : wt);
     const size_t chunks = ((size_t)N + SC_TILE - 1) / SC_TILE;

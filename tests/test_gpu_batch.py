@@ -62,6 +62,21 @@ def test_batch_many_small(gpu):
     check_batch(blocks, rs, "many small")
 
 
+def test_batch_host_inverse_repeated(gpu):
+    """The host-pointer inverse decodes in place in a device buffer that comes back at the same address call after
+    call; each call must read its own input, not lines of the previous call's output still held in a cache."""
+    rng = np.random.default_rng(7)
+    sizes = [int(x) for x in np.exp(rng.uniform(np.log(2), np.log(200000), 300))]
+    blocks = mixed_blocks(300, sizes)
+    rs = [1 << int(rng.integers(1, 12)) for _ in sizes]
+    U, I = K.bwt_aux_batch(blocks, rs)
+    say("host inverse repeated, total", sum(sizes))
+    for rep in range(12):
+        back = K.unbwt_aux_batch(U, rs, I)
+        wrong = [b for b, T in enumerate(blocks) if not np.array_equal(back[b], T)]
+        assert not wrong, f"call {rep}: blocks {wrong[:10]} differ"
+
+
 def test_batch_device_gaps_and_in_place(gpu):
     import torch
     sizes = [70000, 3, 250000, 4096, 1, 33333]
