@@ -1220,9 +1220,8 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     grow(w.cbase, (size_t)B * 256 * 4);
     grow(w.bad, (size_t)B * 4);
     if (B > 1) grow(w.blk, NN * 2);
-    uint8_t *hm = (uint8_t *)host_staging(w, align16(p.meta_bytes) + (size_t)B * 4);
+    uint8_t *hm = (uint8_t *)host_staging(w, align16(p.meta_bytes));
     grow(w.meta, p.meta_bytes);
-    uint32_t *h_bad = (uint32_t *)(hm + align16(p.meta_bytes));
     inverse_meta(p, bl.data(), hm);
     HIPCHK(hipMemcpyAsync(w.meta.p, hm, p.meta_bytes, hipMemcpyHostToDevice, s));
     InvBufs bf{};
@@ -1258,15 +1257,7 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
     run_graph(w.inv_graph, key, s,
               {"fill_blk", "byte_hist", "scan_digits", "lf_pass", "lf_pairs", "walk_count", "rank_jump", "walk_emit"},
               [&] { inverse_launches(p, bf, dL, dOut, &w, s); });
-    LAUNCHCHK();
-    HIPCHK(hipMemcpyAsync(h_bad, w.bad.p, (size_t)B * 4, hipMemcpyDeviceToHost, s));
-    SYNCCHK(s);
-    bool anybad = false;
-    for (uint32_t b = 0; b < B; b++) anybad |= h_bad[b] != 0;
-    if (anybad) {   // strings that are no BWT decode to zeros: deterministic bytes that fail the caller's checksum
-        launch_zero_bad(dOut, bf.blk, bf.bad, N, s);
-        LAUNCHCHK();
-    }
+    LAUNCHCHK();   // a block whose string is no BWT decodes to zeros inside the emission: nothing to read back
     HIPCHK(hipEventRecord(w.ev1, s));
     SYNCCHK(s);
     float ms = 0;
@@ -1281,8 +1272,8 @@ void inverse(Workspace &w, const uint8_t *dL, uint8_t *dOut, std::vector<BlockIn
 // sized for blocks of up to NOALLOC_MAX_N bytes with splitters at least every 8 rows and at most 8 bytes of staged
 // sublist heads per input byte; each device that runs the library holds it until the process ends. In host code the
 // array's name is only a stand-in: its device address comes from hipGetSymbolAddress once per device (with that
-// device current) and only that address is used. Metadata goes up from the caller's stack and the validity flag comes
-// back into it; the launch sequence is the inverse's own (inverse_launches) with lookback epochs counted on the host.
+// device current) and only that address is used. Metadata goes up from the caller's stack; the launch sequence is the
+// inverse's own (inverse_launches) with lookback epochs counted on the host.
 // Calls that overlap on one device share the array and race.
 constexpr uint32_t NOALLOC_MAX_N = 1u << 22;
 constexpr uint32_t NOALLOC_MIN_DSHIFT = 3;
@@ -1388,15 +1379,8 @@ void inverse_noalloc(const uint8_t *dT, uint8_t *dU, uint32_t n, uint32_t primar
     if (!direct) HIPCHK(hipMemcpyAsync(base + NA_T, dT, n, hipMemcpyDeviceToDevice, s));
     launch_fill_u8(base + NA_COUNTERS, 0, 64, s);   // the tile counter starts at zero whatever a failed call left
     HIPCHK(hipMemcpyAsync(bf.meta, hm, p.meta_bytes, hipMemcpyHostToDevice, s));
-    inverse_launches(p, bf, src, dst, nullptr, s);
+    inverse_launches(p, bf, src, dst, nullptr, s);   // a string that is no BWT decodes to zeros
     LAUNCHCHK();
-    uint32_t h_bad = 0;
-    HIPCHK(hipMemcpyAsync(&h_bad, bf.bad, 4, hipMemcpyDeviceToHost, s));
-    SYNCCHK(s);
-    if (h_bad) {   // a string that is no BWT decodes to zeros, as in kbwt_dev_unbwt_aux
-        launch_zero_bad(dst, nullptr, bf.bad, n, s);
-        LAUNCHCHK();
-    }
     if (!direct) HIPCHK(hipMemcpyAsync(dU, base + NA_T, n, hipMemcpyDeviceToDevice, s));
     SYNCCHK(s);
 }

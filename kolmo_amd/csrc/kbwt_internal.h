@@ -173,7 +173,6 @@ void launch_walk_emit(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t d
                       const uint64_t *link, const uint32_t *len, const uint32_t *cbase, const uint8_t *stage,
                       size_t sstride, uint32_t cap, const uint32_t *cont, const uint32_t *bad, uint8_t *out,
                       hipStream_t s);
-void launch_zero_bad(uint8_t *out, const uint16_t *blk, const uint32_t *bad, uint32_t N, hipStream_t s);
 void launch_fill_u8(uint8_t *p, uint8_t v, size_t n, hipStream_t s);
 void launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s);
 

@@ -1373,7 +1373,13 @@ __global__ __launch_bounds__(WALK_TILE) void k_walk_emit(const uint32_t *__restr
     const uint32_t b = tile_block(tl.first, tl.B, blockIdx.x);
     const uint32_t tb = blockIdx.x - tl.first[b];
     const BlockInfo bi = tl.blocks[b];
-    if (bad[b]) return;                                      // uniform over the workgroup
+    if (bad[b]) {   // the string is no BWT: it decodes to zeros (the caller's checksum rejects them)
+        // the block's tiles cover its positions WALK_TILE * D at a time (nsp * D > n); uniform over the workgroup
+        const size_t z0 = ((size_t)tb * WALK_TILE) << dshift;
+        const size_t z1 = z0 + ((size_t)WALK_TILE << dshift) < bi.n ? z0 + ((size_t)WALK_TILE << dshift) : bi.n;
+        for (size_t o = z0 + tid; o < z1; o += WALK_TILE) out[bi.off + o] = 0;
+        return;
+    }
     s_c[tid] = cbase[(size_t)b * 256 + tid];
     if (tid == 0) s_max = 0;
     const uint32_t dmask = (1u << dshift) - 1u;
@@ -1427,13 +1433,6 @@ __global__ __launch_bounds__(WALK_TILE) void k_walk_emit(const uint32_t *__restr
             r = v - bi.roff;
         }
     }
-}
-
-// Blocks whose string is no BWT decode to zeros (the caller's checksum rejects them).
-__global__ __launch_bounds__(256) void k_zero_bad(uint8_t *__restrict__ out, const uint16_t *__restrict__ blk,
-                                                  const uint32_t *__restrict__ bad, uint32_t N) {
-    size_t o = (size_t)blockIdx.x * 256 + threadIdx.x;
-    if (o < N && bad[blk ? blk[o] : 0]) out[o] = 0;
 }
 
 __global__ void k_fill_u8(uint8_t *__restrict__ p, uint8_t v, size_t n) {
@@ -1611,9 +1610,6 @@ void launch_walk_emit(const uint32_t *lf, Tiling tl, uint32_t ntiles, uint32_t d
     if (ntiles)
         k_walk_emit<<<ntiles, WALK_TILE, 0, s>>>(lf, tl, dshift, S, link, len, cbase, stage, sstride, cap, cont, bad,
                                                  out);
-}
-void launch_zero_bad(uint8_t *out, const uint16_t *blk, const uint32_t *bad, uint32_t N, hipStream_t s) {
-    if (N) k_zero_bad<<<(N + 255) / 256, 256, 0, s>>>(out, blk, bad, N);
 }
 void launch_copy16(void *dst, const void *src, size_t bytes, hipStream_t s) {
     size_t n16 = bytes / 16;
