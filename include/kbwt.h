@@ -143,11 +143,11 @@ KBWT_API int32_t kbwt_router_route(const uint8_t *const *blocks, const int32_t *
    tile shape (0: 512 threads x 16 keys with a 4-tile lookback window; 1-10: 256x16, 256x16 with window 1, 512x8,
    256x12, 256x8, 512x16 with window 1, 1024x12, 1024x16, 1024x16 with window 2, 1024x16 with window 8);
    inv_dshift, log2 of the inverse's splitter spacing (2..12); seg_rank, the largest group the doubling rounds rank
-   directly instead of radix-sorting (0..1024). Built-in table: 0, 3, 16 up to 1 MiB; 0, 4, 16 up to 2 MiB; 8, 4, 16
-   up to 8 MiB; 8, 6, 16 above (a call's class is ceil(log2) of its total length). Digits are 8 bits wide in every
-   configuration. None of them changes any output. kbwt_autotune times a forward + inverse of the given device blocks
-   for each candidate, by coordinate descent from start (NULL: the built-in entry; its ms is set to the time at
-   start), installs the fastest for that size class and returns it with its time in best. */
+   directly instead of radix-sorting (0..1024). Built-in table: 0, 3, 64 up to 1 MiB; 0, 4, 64 up to 2 MiB; 8, 4, 64
+   up to 8 MiB; 8, 6, 64 up to 64 MiB; 8, 7, 64 above (a call's class is ceil(log2) of its total length). Digits are
+   8 bits wide in every configuration. None of them changes any output. kbwt_autotune times a forward + inverse of
+   the given device blocks for each candidate, by coordinate descent from start (NULL: the built-in entry; its ms is
+   set to the time at start), installs the fastest for that size class and returns it with its time in best. */
 typedef struct {
     int32_t sort_cfg, inv_dshift, seg_rank;
     float ms;

@@ -2002,7 +2002,7 @@ int32_t kbwt_autotune(const uint8_t *dT, const kbwt_seg *segs, int32_t count, vo
         }
         float cur_ms = trial(cur);
         const float start_ms = cur_ms;
-        const uint32_t dshifts[] = {3, 4, 5, 6, 7}, ranks[] = {0, 8, 16, 64};
+        const uint32_t dshifts[] = {3, 4, 5, 6, 7}, ranks[] = {0, 16, 32, 64, 128};
         for (int pass = 0; pass < 2; pass++) {
             bool moved = false;
             for (uint32_t c = 0; c < (uint32_t)SORT_NCFG; c++) {

@@ -28,7 +28,9 @@ constexpr int WALK_TILE = 256;               // splitters per workgroup of the i
 constexpr uint32_t MIN_TILE = 2048;          // smallest tile of any tiled kernel (sizes the lookback arrays)
 constexpr int SEG_TILE = 1024;               // doubling rounds: rows of the group list per tile (by first row)
 constexpr int SEG_GROUP = 512;               // groups of up to SEG_GROUP rows sort inside one workgroup
-constexpr int SEG_RANK_MAX = 16;             // default: tiles whose groups all have at most this many rows rank directly
+constexpr int SEG_RANK_MAX = 64;             // default: tiles whose groups all have at most this many rows rank directly
+                                             // (O(size) comparisons per row beat two to four LDS radix passes up
+                                             // to 64 rows; at 256 the large groups' quadratic cost dominates)
 
 // One block of a batch as the kernels see it.
 struct BlockInfo {

@@ -135,15 +135,15 @@ def test_generators_deterministic():
 def test_tuning_table_set_get_reset():
     """The per-size-class launch parameters (no device needed): the built-in table, a set, and a reset to it."""
     try:
-        assert K.get_tuning(20) == (0, 3, 16)      # up to 1 MiB
-        assert K.get_tuning(21) == (0, 4, 16)
-        assert K.get_tuning(23) == (8, 4, 16)
-        assert K.get_tuning(25) == (8, 6, 16)      # 32 MiB
-        assert K.get_tuning(27) == (8, 7, 16)      # above 64 MiB
+        assert K.get_tuning(20) == (0, 3, 64)      # up to 1 MiB
+        assert K.get_tuning(21) == (0, 4, 64)
+        assert K.get_tuning(23) == (8, 4, 64)
+        assert K.get_tuning(25) == (8, 6, 64)      # 32 MiB
+        assert K.get_tuning(27) == (8, 7, 64)      # above 64 MiB
         K.set_tuning((5, 7, 64), 20)
-        assert K.get_tuning(20) == (5, 7, 64) and K.get_tuning(21) == (0, 4, 16)
+        assert K.get_tuning(20) == (5, 7, 64) and K.get_tuning(21) == (0, 4, 64)
         K.reset_tuning(20)
-        assert K.get_tuning(20) == (0, 3, 16)
+        assert K.get_tuning(20) == (0, 3, 64)
         K.set_tuning((1, 9, 0))
         assert all(K.get_tuning(c) == (1, 9, 0) for c in range(33))
         with pytest.raises(K.KbwtError):
@@ -152,7 +152,7 @@ def test_tuning_table_set_get_reset():
             K.set_tuning((8, 13, 16))              # splitter spacing out of range
     finally:
         K.reset_tuning()
-    assert K.get_tuning(22) == (8, 4, 16) and K.get_tuning(26) == (8, 6, 16) and K.get_tuning(30) == (8, 7, 16)
+    assert K.get_tuning(22) == (8, 4, 64) and K.get_tuning(26) == (8, 6, 64) and K.get_tuning(30) == (8, 7, 64)
 
 
 def test_concurrent_calls_without_device_fail_cleanly():
