@@ -1451,7 +1451,11 @@ __global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const u
 // waves per SIMD hide that (k_hist_T: 0.131 -> 0.103 ms on a 32 MiB block against 512 workgroups).
 constexpr uint32_t TEXT_HIST_GRID = 1024;
 void launch_byte_hist(const uint8_t *T, Tiling tl, uint32_t ntiles, uint32_t *counts, hipStream_t s) {
-    const uint32_t grid = ntiles < TEXT_HIST_GRID ? ntiles : TEXT_HIST_GRID;
+    // Each workgroup adds its bins to its blocks' 256 counters: more than TEXT_HIST_GRID workgroups on one block
+    // queue on those atomics (0.030 -> 0.046 ms on a 32 MiB block at 2048), while a batch spreads them over its
+    // blocks' counters and gains from more (1.66 -> 1.11 ms per step on 512 blocks at 4096).
+    const uint32_t want = TEXT_HIST_GRID * (tl.B < 4u ? (tl.B ? tl.B : 1u) : 4u);
+    const uint32_t grid = ntiles < want ? ntiles : want;
     if (grid) k_byte_hist<<<grid, 256, 0, s>>>(T, tl, ntiles, counts);
 }
 void launch_scan_digits(uint32_t *counts, uint32_t nsets, uint32_t sets_per_block, const BlockInfo *blocks,
