@@ -77,6 +77,21 @@ def test_batch_host_inverse_repeated(gpu):
         assert not wrong, f"call {rep}: blocks {wrong[:10]} differ"
 
 
+def test_batch_not_a_bwt_block(gpu):
+    """A block whose string is no BWT decodes to zeros over its whole length (the emission zeroes it tile by tile,
+    about 150 tiles here), and the blocks around it in the same batch still decode exactly."""
+    sizes = [50000, 300000, 7000, 123457]
+    blocks = mixed_blocks(500, sizes)
+    rs = [1 << K.codec18_rlog(n) for n in sizes]
+    U, I = K.bwt_aux_batch(blocks, rs)
+    U[1] = np.frombuffer(b"aabb" * 75000, np.uint8).copy()   # its LF map has more than one cycle
+    say("batch with a block that is no BWT", sizes)
+    back = K.unbwt_aux_batch(U, rs, I)
+    assert back[1].size == sizes[1] and not back[1].any()
+    for b in (0, 2, 3):
+        assert np.array_equal(back[b], blocks[b]), b
+
+
 def test_batch_device_gaps_and_in_place(gpu):
     import torch
     sizes = [70000, 3, 250000, 4096, 1, 33333]

@@ -155,6 +155,7 @@ def test_gpu_not_a_bwt(gpu):
     assert back.size == U.size
     back2 = K.unbwt_aux(U, U.size, np.array([1], np.int32))
     assert np.array_equal(back, back2)
+    assert not back.any()                                  # every byte zero
 
 
 @pytest.mark.parametrize("rank_max", ["0", "2", "1024"])
