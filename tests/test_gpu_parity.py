@@ -147,6 +147,18 @@ def test_gpu_device_resident_api(gpu):
     assert np.array_equal(dX.cpu().numpy(), T)
 
 
+def test_gpu_isa_window_edges(gpu):
+    """Single blocks around the edges of the first round's ISA assembly: 32 K-suffix workgroup ranges that end inside
+    or exactly at a block's end, partition windows narrower than, as wide as and twice or four times as wide as a
+    workgroup's range (blocks up to 8 MiB, to 16 MiB, to 32 MiB), and sizes one off those edges."""
+    for kind, n in [("mixed", 32767), ("mixed", 32768), ("mixed", 32769), ("dna", 98305), ("logs", 131071),
+                    ("csvcols", 131073), ("mixed", (1 << 23) - 1), ("markov2", (1 << 23) + 1),
+                    ("repeats", (1 << 24) + 12345)]:
+        T = K.generate(kind, 77, n)
+        say("isa window edges", kind, n)
+        check_case(T, 1 << K.codec18_rlog(n), f"isa edges {kind} n {n}")
+
+
 def test_gpu_not_a_bwt(gpu):
     """A byte string that is no BWT (several LF cycles) decodes to defined bytes with code 0, as kolmo's decoder
     returns 0 there too; the caller's checksum rejects it."""
